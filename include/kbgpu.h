@@ -1,0 +1,133 @@
+/*
+ * kbgpu.h -- C ABI of libkbgpu.so, the MI355X (gfx950) discrete-velocity time stepper that replaces
+ * KitBase.jl's CPU `reconstruct!` / `evolve!` / `update!` hot loop.
+ *
+ * Plain pointers and sizes only.  All host arrays are FP64, contiguous, owned by the caller, and
+ * laid out the way KitBase stores the same data once its OffsetArrays are flattened:
+ *   1D  cells  k = i - 1 + ng for the physical index i in [1-ng, nx+ng]  (N = nx + 2 ng rows)
+ *       faces  k - 1 for the physical face k in [1, nx+1] (between cells k-1 and k)
+ *   2D  cells  (i-1) + nx (j-1);  x-faces (i-1) + (nx+1)(j-1), i in [1, nx+1];
+ *       y-faces (i-1) + nx (j-1), j in [1, ny+1]
+ *   per cell:  w, prim [nw];  f [nf][nv] (h then b);  sw [nw] (2D: [2 dirs][nw]);
+ *              sf [nf][nv] (2D: [2 dirs][nf][nv])
+ *   per face:  fw [nw];  ff [nf][nv]
+ *   velocity nodes u-fastest (Julia column-major), nv points in total.
+ * nw = 3 (1D1V), 4 (2D2V), 5 (1D3V).  prim = [rho, U, (V), (W), lambda], lambda = 1/(2RT).
+ *
+ * Every call returns KBG_OK (0) or a negative KBG_ERR_* code; kbg_last_error() then describes the
+ * failure (thread-local).  Calls on one kbg_solve must come from one host thread at a time; calls
+ * are ordered on the context's HIP stream and return once their results are visible to the host
+ * (downloads, residuals) or queued (steps).  There is no CPU fallback: a missing or unusable GPU is
+ * an error.
+ *
+ * Citations are relative to the KitBase.jl v0.9.31 source tree.
+ */
+#ifndef KBGPU_H
+#define KBGPU_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KBG_OK 0
+#define KBG_ERR_ARG (-1)
+#define KBG_ERR_HIP (-2)
+#define KBG_ERR_UNSUPPORTED (-3)
+#define KBG_ERR_STATE (-4)
+#define KBG_ERR_NOMEM (-5)
+
+enum { KBG_FP32 = 0, KBG_FP64 = 1 };
+/* flux symbols of evolve!(...; mode) -- solver_evolution.jl:55 evaluates "flux_" * mode * "!" */
+enum { KBG_KFVS = 0, KBG_KCU = 1, KBG_UGKS = 2 };
+/* collision symbols of update!(...; coll) -- step_1f.jl:30, step_2f.jl:34 */
+enum { KBG_BGK = 0, KBG_SHAKHOV = 1 };
+/* boundary symbols: :fix (boundary cells not stepped) and :maxwell (diffuse walls) */
+enum { KBG_BC_FIX = 0, KBG_BC_MAXWELL = 1 };
+/* limiter symbols -- recon_limiter.jl:10-105 */
+enum { KBG_VANLEER = 0, KBG_MINMOD = 1, KBG_SUPERBEE = 2, KBG_VANALBABA = 3, KBG_LINEAR = 4 };
+
+typedef struct kbg_ctx kbg_ctx;
+typedef struct kbg_solve kbg_solve;
+
+/* One solve: the fields of KitBase's SolverSet that the time step reads (solver_set.jl:10-117:
+ * Setup struct_setup.jl:106-119, PSpace1D/2D geo_struct.jl:10-175, VSpace1D/2D/3D
+ * phase_velocity.jl:14-258, Gas struct_property.jl:51-64, the IB boundary primitive vectors). */
+typedef struct kbg_desc {
+    int dim;          /* 1 or 2 */
+    int nf;           /* 1 (f) or 2 (h, b) */
+    int vdim;         /* velocity dimension: 1 or 3 in 1D, 2 in 2D */
+    int nv;           /* total velocity points */
+    int nx, ny;       /* physical cells (ny = 1 in 1D) */
+    int ng;           /* ghost layers (1D; must be 0 in 2D) */
+    int precision;    /* KBG_FP32 or KBG_FP64 storage of f and the per-velocity arithmetic */
+    int flux;         /* KBG_KFVS, KBG_KCU, KBG_UGKS */
+    int collision;    /* KBG_BGK, KBG_SHAKHOV */
+    int limiter;      /* KBG_VANLEER, ... */
+    int interp_order; /* 1 or 2 (Setup.interpOrder) */
+    int bc[4];        /* 1D: left, right; 2D: left, right, down, up -- KBG_BC_FIX / KBG_BC_MAXWELL */
+    double K, gamma, mu_ref, omega, Pr; /* Gas.K, Gas.γ, Gas.μᵣ, Gas.ω, Gas.Pr */
+    const double *u, *v, *w, *weights;  /* [nv]; v and w may be NULL when unused */
+    const double *dx;                   /* 1D: [nx + 2 ng]; 2D: [nx*ny] */
+    const double *dy;                   /* 2D: [nx*ny] */
+    const double *bc_prim;              /* 1D Maxwell walls: [2][nw] prim at x0 and x1 (KS.ib.bc) */
+    const double *xgeo, *ygeo;          /* 2D: per-face [n1, n2, len] (geo_struct.jl normals/areas) */
+    const double *xwall, *ywall;        /* 2D: per-face wall prim [4] in the global frame */
+} kbg_desc;
+
+/* Library version (major*10000 + minor*100 + patch) and the calling thread's last error. */
+int kbg_version(void);
+const char *kbg_last_error(void);
+
+/* Context: one device, one stream, tuning state.  No counterpart in KitBase (it has no device). */
+int kbg_ctx_create(int device, kbg_ctx **out);
+int kbg_ctx_destroy(kbg_ctx *ctx);
+int kbg_ctx_sync(kbg_ctx *ctx);
+/* Fused-kernel schedule: cells per work item and wavefronts launched (0 = one per item). */
+int kbg_ctx_set_schedule(kbg_ctx *ctx, int seg_cells, int waves);
+int kbg_ctx_get_schedule(kbg_ctx *ctx, int *seg_cells, int *waves);
+/* HIP events on the context stream, for timing: record slot (0..15), elapsed ms between slots. */
+int kbg_ctx_event_record(kbg_ctx *ctx, int slot);
+int kbg_ctx_event_elapsed(kbg_ctx *ctx, int slot_a, int slot_b, float *ms);
+
+/* A device-resident solve: replaces the arrays of ControlVolume1F/2F (struct_ctr.jl:51-241) and
+ * Interface1F/2F (struct_face.jl:37-101) built by init_fvm (solver_init.jl:62-327). */
+int kbg_solve_create(kbg_ctx *ctx, const kbg_desc *desc, kbg_solve **out);
+int kbg_solve_destroy(kbg_solve *s);
+/* Device bytes a solve with this description occupies on the fused path (admission control). */
+size_t kbg_solve_bytes(const kbg_desc *desc);
+
+/* Host <-> device copies of the cell and face state (FP64 on the host, converted on the fly). */
+int kbg_upload(kbg_solve *s, const double *w, const double *prim, const double *f);
+int kbg_download(kbg_solve *s, double *w, double *prim, double *f);
+int kbg_upload_slopes(kbg_solve *s, const double *sw, const double *sf);
+int kbg_download_slopes(kbg_solve *s, double *sw, double *sf);
+/* 2D: x-faces first, then y-faces, in both arrays. */
+int kbg_upload_faces(kbg_solve *s, const double *fw, const double *ff);
+int kbg_download_faces(kbg_solve *s, double *fw, double *ff);
+
+/* reconstruct!(KS, ctr)                  solver_reconstruction.jl:56-153 (1D), :475-620 (2D) */
+int kbg_reconstruct(kbg_solve *s);
+/* evolve!(KS, ctr, face, dt; mode, bc)   solver_evolution.jl:39-89 (1D), :608-679 (2D),
+ *                                        evolve_boundary.jl:4-178 (Maxwell walls) */
+int kbg_evolve(kbg_solve *s, double dt);
+/* update!(KS, ctr, face, dt, residual; coll, bc)  solver_update.jl:61-93 (1D), :193-246 (2D),
+ * update_boundary.jl:6-306.  residual: [nw] or NULL. */
+int kbg_update(kbg_solve *s, double dt, double *residual);
+
+/* nsteps iterations of solve!'s loop body (solver_main.jl:32-51) with a fixed dt, fused on the
+ * device; residual (of the last step) may be NULL.  Slopes and face arrays are not kept. */
+int kbg_step(kbg_solve *s, double dt, int nsteps, double *residual);
+
+/* One solve!-loop iteration for every solve of a batch, nsteps times: the 1D solves share one
+ * fused launch per step; the others are stepped in turn on the same stream.  dts[i] is solve i's
+ * time step; residuals (n x 5, may be NULL) receive the last step's residuals. */
+int kbg_batch_step(kbg_ctx *ctx, kbg_solve *const *solves, int n, const double *dts, int nsteps,
+                   double *residuals);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KBGPU_H */

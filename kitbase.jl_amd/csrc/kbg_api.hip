@@ -1,0 +1,871 @@
+// kbg_api.hip -- the C ABI of libkbgpu.so (declared in include/kbgpu.h): contexts, device-resident
+// solves, host<->device conversion, the phase-exact reconstruct/evolve/update entry points, and the
+// fused single-solve and batched time steps.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/kbgpu.h"
+#include "kbg_types.h"
+
+using kbg::Solve1D;
+using kbg::Solve2D;
+using kbg::WorkItem;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                                   \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess) return fail(KBG_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr int kMaxVpt = 4;    // fused 1D kernel: up to 4 velocities per lane (nv <= 256)
+constexpr int kMaxSeg = 252;  // fused 1D kernel: cells per work item (LDS slice bound)
+
+}  // namespace
+
+struct kbg_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[16] = {};
+    int seg_cells = 64;
+    int waves = 0;
+    // batch staging (device)
+    Solve1D* d_descs = nullptr;
+    size_t descs_cap = 0;
+    WorkItem* d_items = nullptr;
+    size_t items_cap = 0;
+    double* d_rpart = nullptr;
+    size_t rpart_cap = 0;
+    double* d_res = nullptr;  // [4096][5]
+};
+
+struct kbg_solve {
+    kbg_ctx* ctx = nullptr;
+    kbg_desc d{};
+    bool fp64 = false;
+    size_t esz = 4;
+    int nw = 3, N = 0, RS = 0;
+    bool fused = false;
+    // common device buffers
+    void *u = nullptr, *v = nullptr, *wv = nullptr, *wt = nullptr;
+    double *dx = nullptr, *dy = nullptr;
+    double* rcell = nullptr;
+    double* res = nullptr;
+    // 1D
+    Solve1D h1{};
+    Solve1D* d1 = nullptr;
+    std::vector<WorkItem> items;
+    int items_seg = 0;
+    WorkItem* d_items = nullptr;
+    double* d_rpart = nullptr;
+    // 2D
+    Solve2D h2{};
+    Solve2D* d2 = nullptr;
+    std::vector<void*> allocs;
+};
+
+namespace {
+
+int dev_alloc(kbg_solve* s, void** p, size_t bytes) {
+    if (bytes == 0) bytes = 8;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) return fail(KBG_ERR_NOMEM, std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e));
+    s->allocs.push_back(*p);
+    return KBG_OK;
+}
+
+// copy n doubles to device as the solve's storage type
+int put_vec(kbg_solve* s, void* dst, const double* src, size_t n) {
+    if (s->fp64) {
+        HIPCHK(hipMemcpyAsync(dst, src, n * 8, hipMemcpyHostToDevice, s->ctx->stream));
+        HIPCHK(hipStreamSynchronize(s->ctx->stream));
+    } else {
+        std::vector<float> tmp(n);
+        for (size_t i = 0; i < n; i++) tmp[i] = float(src[i]);
+        HIPCHK(hipMemcpyAsync(dst, tmp.data(), n * 4, hipMemcpyHostToDevice, s->ctx->stream));
+        HIPCHK(hipStreamSynchronize(s->ctx->stream));
+    }
+    return KBG_OK;
+}
+
+int get_vec(kbg_solve* s, double* dst, const void* src, size_t n) {
+    if (s->fp64) {
+        HIPCHK(hipMemcpyAsync(dst, src, n * 8, hipMemcpyDeviceToHost, s->ctx->stream));
+        HIPCHK(hipStreamSynchronize(s->ctx->stream));
+    } else {
+        std::vector<float> tmp(n);
+        HIPCHK(hipMemcpyAsync(tmp.data(), src, n * 4, hipMemcpyDeviceToHost, s->ctx->stream));
+        HIPCHK(hipStreamSynchronize(s->ctx->stream));
+        for (size_t i = 0; i < n; i++) dst[i] = double(tmp[i]);
+    }
+    return KBG_OK;
+}
+
+int put_f64(kbg_solve* s, double* dst, const double* src, size_t n) {
+    HIPCHK(hipMemcpyAsync(dst, src, n * 8, hipMemcpyHostToDevice, s->ctx->stream));
+    HIPCHK(hipStreamSynchronize(s->ctx->stream));
+    return KBG_OK;
+}
+
+int get_f64(kbg_solve* s, double* dst, const double* src, size_t n) {
+    HIPCHK(hipMemcpyAsync(dst, src, n * 8, hipMemcpyDeviceToHost, s->ctx->stream));
+    HIPCHK(hipStreamSynchronize(s->ctx->stream));
+    return KBG_OK;
+}
+
+int validate(const kbg_desc* d) {
+    if (!d) return fail(KBG_ERR_ARG, "desc is NULL");
+    if (d->precision != KBG_FP32 && d->precision != KBG_FP64) return fail(KBG_ERR_ARG, "precision must be KBG_FP32 or KBG_FP64");
+    if (d->flux < KBG_KFVS || d->flux > KBG_UGKS) return fail(KBG_ERR_ARG, "unknown flux");
+    if (d->collision != KBG_BGK && d->collision != KBG_SHAKHOV) return fail(KBG_ERR_ARG, "unknown collision");
+    if (d->limiter < KBG_VANLEER || d->limiter > KBG_LINEAR) return fail(KBG_ERR_ARG, "unknown limiter");
+    if (d->interp_order != 1 && d->interp_order != 2) return fail(KBG_ERR_ARG, "interp_order must be 1 or 2");
+    if (d->nv < 1) return fail(KBG_ERR_ARG, "nv must be positive");
+    if (!d->u || !d->weights || !d->dx) return fail(KBG_ERR_ARG, "u, weights and dx are required");
+    if (d->dim == 1) {
+        if (d->nx < 3) return fail(KBG_ERR_ARG, "1D needs nx >= 3");
+        if (d->ng < 0) return fail(KBG_ERR_ARG, "ng must be >= 0");
+        if (d->vdim == 1) {
+            if (d->nf != 1 && d->nf != 2) return fail(KBG_ERR_ARG, "1D1V needs nf = 1 or 2");
+            if (d->flux == KBG_UGKS && d->nf != 2) return fail(KBG_ERR_UNSUPPORTED, "KitBase has no 1D1F1V UGKS flux (flux_ugks.jl covers 1D2F1V)");
+        } else if (d->vdim == 3) {
+            if (d->nf != 1) return fail(KBG_ERR_UNSUPPORTED, "1D3V supports nf = 1 (1D1F3V)");
+            if (!d->v || !d->w) return fail(KBG_ERR_ARG, "1D3V needs v and w");
+            if (d->flux == KBG_UGKS) return fail(KBG_ERR_UNSUPPORTED, "KitBase has no 1D1F3V UGKS flux");
+            if (d->collision == KBG_SHAKHOV)
+                return fail(KBG_ERR_UNSUPPORTED, "1D1F3V Shakhov step! references an undefined K (step_1f.jl:91); only BGK is defined");
+        } else {
+            return fail(KBG_ERR_ARG, "1D needs vdim 1 or 3");
+        }
+        for (int sd = 0; sd < 2; sd++)
+            if (d->bc[sd] != KBG_BC_FIX && d->bc[sd] != KBG_BC_MAXWELL) return fail(KBG_ERR_UNSUPPORTED, "1D boundaries: fix or maxwell");
+        if (d->nf == 2 && (d->bc[0] == KBG_BC_MAXWELL || d->bc[1] == KBG_BC_MAXWELL) && !d->bc_prim)
+            return fail(KBG_ERR_ARG, "Maxwell walls need bc_prim");
+    } else if (d->dim == 2) {
+        if (d->nf != 2 || d->vdim != 2) return fail(KBG_ERR_UNSUPPORTED, "2D supports 2F2V");
+        if (d->nx < 3 || d->ny < 3) return fail(KBG_ERR_ARG, "2D needs nx, ny >= 3");
+        if (d->ng != 0) return fail(KBG_ERR_UNSUPPORTED, "2D ghost layers are not supported");
+        if (!d->v || !d->dy || !d->xgeo || !d->ygeo) return fail(KBG_ERR_ARG, "2D needs v, dy, xgeo, ygeo");
+        for (int sd = 0; sd < 4; sd++) {
+            if (d->bc[sd] != KBG_BC_FIX && d->bc[sd] != KBG_BC_MAXWELL) return fail(KBG_ERR_UNSUPPORTED, "2D boundaries: fix or maxwell");
+            if (d->bc[sd] == KBG_BC_MAXWELL && (!d->xwall || !d->ywall)) return fail(KBG_ERR_ARG, "Maxwell walls need xwall and ywall");
+        }
+    } else {
+        return fail(KBG_ERR_ARG, "dim must be 1 or 2");
+    }
+    return KBG_OK;
+}
+
+int nw_of(const kbg_desc* d) { return d->dim == 2 ? 4 : (d->vdim == 3 ? 5 : 3); }
+
+void fill1d(kbg_solve* s) {
+    const kbg_desc& d = s->d;
+    Solve1D& S = s->h1;
+    S.nx = d.nx;
+    S.ng = d.ng;
+    S.nv = d.nv;
+    S.nf = d.nf;
+    S.vdim = d.vdim;
+    S.nw = s->nw;
+    S.flux = d.flux;
+    S.coll = d.collision;
+    S.limiter = d.limiter;
+    S.interp2 = d.interp_order == 2;
+    S.upd_lo = d.bc[0] == KBG_BC_FIX ? 2 : 1;
+    S.upd_hi = d.bc[1] == KBG_BC_FIX ? d.nx - 1 : d.nx;
+    if (S.interp2) {
+        S.slope_lo = d.ng > 0 ? 1 : 2;
+        S.slope_hi = d.ng > 0 ? d.nx : d.nx - 1;
+    } else {
+        S.slope_lo = 1;
+        S.slope_hi = 0;
+    }
+    S.face_lo = d.ng > 0 ? 1 : 2;
+    S.face_hi = d.ng > 0 ? d.nx + 1 : d.nx;
+    S.wallL = d.nf == 2 && d.bc[0] == KBG_BC_MAXWELL;
+    S.wallR = d.nf == 2 && d.bc[1] == KBG_BC_MAXWELL;
+    S.faceL_buf = d.ng == 0 || S.wallL;
+    S.faceR_buf = d.ng == 0 || S.wallR;
+    S.cur = 0;
+    S.K = d.K;
+    S.gam = d.gamma;
+    S.mu = d.mu_ref;
+    S.om = d.omega;
+    S.Pr = d.Pr;
+    S.dt = 0.0;
+}
+
+void fill2d(kbg_solve* s) {
+    const kbg_desc& d = s->d;
+    Solve2D& S = s->h2;
+    S.nx = d.nx;
+    S.ny = d.ny;
+    S.nv = d.nv;
+    S.flux = d.flux;
+    S.coll = d.collision;
+    S.limiter = d.limiter;
+    S.interp2 = d.interp_order == 2;
+    for (int k = 0; k < 4; k++) S.bc[k] = d.bc[k];
+    S.K = d.K;
+    S.gam = d.gamma;
+    S.mu = d.mu_ref;
+    S.om = d.omega;
+    S.Pr = d.Pr;
+    S.dt = 0.0;
+}
+
+int push1d(kbg_solve* s) {
+    HIPCHK(hipMemcpyAsync(s->d1, &s->h1, sizeof(Solve1D), hipMemcpyHostToDevice, s->ctx->stream));
+    HIPCHK(hipStreamSynchronize(s->ctx->stream));
+    return KBG_OK;
+}
+
+int push2d(kbg_solve* s) {
+    HIPCHK(hipMemcpyAsync(s->d2, &s->h2, sizeof(Solve2D), hipMemcpyHostToDevice, s->ctx->stream));
+    HIPCHK(hipStreamSynchronize(s->ctx->stream));
+    return KBG_OK;
+}
+
+// slope / face buffers of the phase-exact path, allocated on first use
+int ensure_phase_buffers(kbg_solve* s) {
+    int rc;
+    if (s->d.dim == 1) {
+        Solve1D& S = s->h1;
+        if (S.sw) return KBG_OK;
+        const size_t N = s->N, nf1 = size_t(s->d.nx) + 1;
+        void* p;
+        if ((rc = dev_alloc(s, &p, N * s->nw * 8))) return rc;
+        S.sw = (double*)p;
+        HIPCHK(hipMemsetAsync(S.sw, 0, N * s->nw * 8, s->ctx->stream));
+        if ((rc = dev_alloc(s, &p, N * s->RS * s->esz))) return rc;
+        S.sf = p;
+        HIPCHK(hipMemsetAsync(S.sf, 0, N * s->RS * s->esz, s->ctx->stream));
+        if ((rc = dev_alloc(s, &p, nf1 * s->nw * 8))) return rc;
+        S.fw = (double*)p;
+        HIPCHK(hipMemsetAsync(S.fw, 0, nf1 * s->nw * 8, s->ctx->stream));
+        if ((rc = dev_alloc(s, &p, nf1 * s->RS * s->esz))) return rc;
+        S.ff = p;
+        HIPCHK(hipMemsetAsync(S.ff, 0, nf1 * s->RS * s->esz, s->ctx->stream));
+        // the stored faces 1 and nx+1 start from the fused path's boundary-face buffers
+        HIPCHK(hipMemcpyAsync(S.fw, S.bfw, s->nw * 8, hipMemcpyDeviceToDevice, s->ctx->stream));
+        HIPCHK(hipMemcpyAsync(S.fw + (nf1 - 1) * s->nw, S.bfw + s->nw, s->nw * 8, hipMemcpyDeviceToDevice, s->ctx->stream));
+        HIPCHK(hipMemcpyAsync(S.ff, S.bff, s->RS * s->esz, hipMemcpyDeviceToDevice, s->ctx->stream));
+        HIPCHK(hipMemcpyAsync((char*)S.ff + (nf1 - 1) * s->RS * s->esz, (char*)S.bff + s->RS * s->esz, s->RS * s->esz,
+                              hipMemcpyDeviceToDevice, s->ctx->stream));
+        return push1d(s);
+    }
+    return KBG_OK;
+}
+
+// boundary-face buffers follow the stored faces 1 and nx+1 after phase-level calls
+int sync_boundary_faces(kbg_solve* s) {
+    Solve1D& S = s->h1;
+    if (!S.fw) return KBG_OK;
+    const size_t nf1 = size_t(s->d.nx) + 1;
+    HIPCHK(hipMemcpyAsync(S.bfw, S.fw, s->nw * 8, hipMemcpyDeviceToDevice, s->ctx->stream));
+    HIPCHK(hipMemcpyAsync(S.bfw + s->nw, S.fw + (nf1 - 1) * s->nw, s->nw * 8, hipMemcpyDeviceToDevice, s->ctx->stream));
+    HIPCHK(hipMemcpyAsync(S.bff, S.ff, s->RS * s->esz, hipMemcpyDeviceToDevice, s->ctx->stream));
+    HIPCHK(hipMemcpyAsync((char*)S.bff + s->RS * s->esz, (char*)S.ff + (nf1 - 1) * s->RS * s->esz, s->RS * s->esz,
+                          hipMemcpyDeviceToDevice, s->ctx->stream));
+    return KBG_OK;
+}
+
+void build_items(kbg_solve* s, int seg, int solve_index, std::vector<WorkItem>& out) {
+    const Solve1D& S = s->h1;
+    const int vpt = (S.nv + 63) / 64;
+    for (int c0 = S.upd_lo; c0 <= S.upd_hi; c0 += seg) {
+        WorkItem wi;
+        wi.solve = solve_index;
+        wi.c0 = c0;
+        wi.c1 = std::min(c0 + seg - 1, S.upd_hi);
+        wi.vpt = vpt;
+        out.push_back(wi);
+    }
+}
+
+int grid_for(kbg_ctx* c, int nitems) {
+    int waves = c->waves > 0 ? std::min(c->waves, nitems) : nitems;
+    return std::max(1, (waves + 3) / 4);
+}
+
+int step_phase_1d(kbg_solve* s, double dt, double* residual);
+int step_phase_2d(kbg_solve* s, double dt, double* residual);
+
+}  // namespace
+
+extern "C" {
+
+int kbg_version(void) { return 1 * 10000 + 0 * 100 + 0; }
+
+const char* kbg_last_error(void) { return g_err.c_str(); }
+
+int kbg_ctx_create(int device, kbg_ctx** out) {
+    if (!out) return fail(KBG_ERR_ARG, "out is NULL");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0) return fail(KBG_ERR_HIP, "no HIP device available");
+    if (device < 0 || device >= n) return fail(KBG_ERR_ARG, "device index out of range");
+    HIPCHK(hipSetDevice(device));
+    kbg_ctx* c = new kbg_ctx();
+    c->device = device;
+    e = hipStreamCreate(&c->stream);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(KBG_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    for (auto& ev : c->ev) HIPCHK(hipEventCreate(&ev));
+    HIPCHK(hipMalloc((void**)&c->d_res, 4096 * 5 * sizeof(double)));
+    *out = c;
+    return KBG_OK;
+}
+
+int kbg_ctx_destroy(kbg_ctx* c) {
+    if (!c) return KBG_OK;
+    (void)hipStreamSynchronize(c->stream);
+    for (auto& ev : c->ev) (void)hipEventDestroy(ev);
+    (void)hipFree(c->d_descs);
+    (void)hipFree(c->d_items);
+    (void)hipFree(c->d_rpart);
+    (void)hipFree(c->d_res);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return KBG_OK;
+}
+
+int kbg_ctx_sync(kbg_ctx* c) {
+    if (!c) return fail(KBG_ERR_ARG, "ctx is NULL");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return KBG_OK;
+}
+
+int kbg_ctx_set_schedule(kbg_ctx* c, int seg_cells, int waves) {
+    if (!c) return fail(KBG_ERR_ARG, "ctx is NULL");
+    // the fused kernel stages (seg_cells + 4) x 8 doubles per wavefront in LDS: 4 wavefronts per
+    // workgroup keep that under 64 KiB
+    if (seg_cells < 1 || seg_cells > kMaxSeg || waves < 0)
+        return fail(KBG_ERR_ARG, "schedule: seg_cells must be in [1, " + std::to_string(kMaxSeg) + "], waves >= 0");
+    c->seg_cells = seg_cells;
+    c->waves = waves;
+    return KBG_OK;
+}
+
+int kbg_ctx_get_schedule(kbg_ctx* c, int* seg_cells, int* waves) {
+    if (!c) return fail(KBG_ERR_ARG, "ctx is NULL");
+    if (seg_cells) *seg_cells = c->seg_cells;
+    if (waves) *waves = c->waves;
+    return KBG_OK;
+}
+
+int kbg_ctx_event_record(kbg_ctx* c, int slot) {
+    if (!c || slot < 0 || slot >= 16) return fail(KBG_ERR_ARG, "bad event slot");
+    HIPCHK(hipEventRecord(c->ev[slot], c->stream));
+    return KBG_OK;
+}
+
+int kbg_ctx_event_elapsed(kbg_ctx* c, int a, int b, float* ms) {
+    if (!c || a < 0 || a >= 16 || b < 0 || b >= 16 || !ms) return fail(KBG_ERR_ARG, "bad event slot");
+    HIPCHK(hipEventSynchronize(c->ev[b]));
+    HIPCHK(hipEventElapsedTime(ms, c->ev[a], c->ev[b]));
+    return KBG_OK;
+}
+
+size_t kbg_solve_bytes(const kbg_desc* d) {
+    if (!d) return 0;
+    const size_t esz = d->precision == KBG_FP64 ? 8 : 4;
+    const int nw = nw_of(d);
+    if (d->dim == 1) {
+        const size_t N = size_t(d->nx) + 2 * size_t(d->ng);
+        return 2 * N * d->nf * d->nv * esz + 3 * N * nw * 8 + N * 8 + 4 * size_t(d->nv) * esz + 2 * d->nf * d->nv * esz;
+    }
+    const size_t C = size_t(d->nx) * d->ny, F = size_t(d->nx + 1) * d->ny + size_t(d->nx) * (d->ny + 1);
+    return C * 2 * d->nv * esz * 3 + F * 2 * d->nv * esz + C * (2 * 4 * 8 + 8 * 8 + 16 * 8 + 16) + F * (4 + 3 + 4) * 8 +
+           3 * size_t(d->nv) * esz;
+}
+
+int kbg_solve_create(kbg_ctx* ctx, const kbg_desc* desc, kbg_solve** out) {
+    if (!ctx || !out) return fail(KBG_ERR_ARG, "ctx or out is NULL");
+    int rc = validate(desc);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(ctx->device));
+    kbg_solve* s = new kbg_solve();
+    s->ctx = ctx;
+    s->d = *desc;
+    s->fp64 = desc->precision == KBG_FP64;
+    s->esz = s->fp64 ? 8 : 4;
+    s->nw = nw_of(desc);
+    const kbg_desc& d = s->d;
+    s->RS = d.nf * d.nv;
+    auto bail = [&](int code) {
+        kbg_solve_destroy(s);
+        return code;
+    };
+    void* p;
+    // velocity grid
+    if ((rc = dev_alloc(s, &s->u, d.nv * s->esz)) || (rc = put_vec(s, s->u, d.u, d.nv))) return bail(rc);
+    if ((rc = dev_alloc(s, &s->wt, d.nv * s->esz)) || (rc = put_vec(s, s->wt, d.weights, d.nv))) return bail(rc);
+    if (d.v) {
+        if ((rc = dev_alloc(s, &s->v, d.nv * s->esz)) || (rc = put_vec(s, s->v, d.v, d.nv))) return bail(rc);
+    }
+    if (d.w && d.dim == 1 && d.vdim == 3) {
+        if ((rc = dev_alloc(s, &s->wv, d.nv * s->esz)) || (rc = put_vec(s, s->wv, d.w, d.nv))) return bail(rc);
+    }
+    if (d.dim == 1) {
+        s->N = d.nx + 2 * d.ng;
+        const size_t N = s->N;
+        fill1d(s);
+        Solve1D& S = s->h1;
+        if ((rc = dev_alloc(s, &p, N * 8))) return bail(rc);
+        s->dx = (double*)p;
+        if ((rc = put_f64(s, s->dx, d.dx, N))) return bail(rc);
+        for (int b = 0; b < 2; b++) {
+            if ((rc = dev_alloc(s, &p, N * s->nw * 8))) return bail(rc);
+            S.w[b] = (double*)p;
+            HIPCHK(hipMemset(S.w[b], 0, N * s->nw * 8));
+            if ((rc = dev_alloc(s, &p, N * s->RS * s->esz))) return bail(rc);
+            S.f[b] = p;
+            HIPCHK(hipMemset(S.f[b], 0, N * s->RS * s->esz));
+        }
+        if ((rc = dev_alloc(s, &p, N * s->nw * 8))) return bail(rc);
+        S.prim = (double*)p;
+        HIPCHK(hipMemset(S.prim, 0, N * s->nw * 8));
+        if ((rc = dev_alloc(s, &p, 2 * s->nw * 8))) return bail(rc);
+        S.bfw = (double*)p;
+        HIPCHK(hipMemset(S.bfw, 0, 2 * s->nw * 8));
+        if ((rc = dev_alloc(s, &p, 2 * s->RS * s->esz))) return bail(rc);
+        S.bff = p;
+        HIPCHK(hipMemset(S.bff, 0, 2 * s->RS * s->esz));
+        if ((rc = dev_alloc(s, &p, 2 * s->nw * 8))) return bail(rc);
+        S.bcprim = (const double*)p;
+        if (d.bc_prim) {
+            if ((rc = put_f64(s, (double*)p, d.bc_prim, 2 * s->nw))) return bail(rc);
+        } else {
+            HIPCHK(hipMemset(p, 0, 2 * s->nw * 8));
+        }
+        S.dx = s->dx;
+        S.u = s->u;
+        S.v = s->v;
+        S.wv = s->wv;
+        S.wt = s->wt;
+        if ((rc = dev_alloc(s, &p, N * 4 * s->nw * 8))) return bail(rc);
+        s->rcell = (double*)p;
+        if ((rc = dev_alloc(s, &p, 8 * 8))) return bail(rc);
+        s->res = (double*)p;
+        if ((rc = dev_alloc(s, &p, sizeof(Solve1D)))) return bail(rc);
+        s->d1 = (Solve1D*)p;
+        s->fused = d.vdim == 1 && d.nv <= 64 * kMaxVpt;
+        if ((rc = push1d(s))) return bail(rc);
+    } else {
+        const size_t C = size_t(d.nx) * d.ny, FX = size_t(d.nx + 1) * d.ny, FY = size_t(d.nx) * (d.ny + 1);
+        s->N = (int)C;
+        fill2d(s);
+        Solve2D& S = s->h2;
+        double* dd;
+        if ((rc = dev_alloc(s, &p, C * 8)) || (rc = put_f64(s, (double*)p, d.dx, C))) return bail(rc);
+        s->dx = (double*)p;
+        if ((rc = dev_alloc(s, &p, C * 8)) || (rc = put_f64(s, (double*)p, d.dy, C))) return bail(rc);
+        s->dy = (double*)p;
+        if ((rc = dev_alloc(s, &p, FX * 3 * 8)) || (rc = put_f64(s, (double*)p, d.xgeo, FX * 3))) return bail(rc);
+        S.xgeo = (const double*)p;
+        if ((rc = dev_alloc(s, &p, FY * 3 * 8)) || (rc = put_f64(s, (double*)p, d.ygeo, FY * 3))) return bail(rc);
+        S.ygeo = (const double*)p;
+        if ((rc = dev_alloc(s, &p, FX * 4 * 8))) return bail(rc);
+        dd = (double*)p;
+        if (d.xwall) {
+            if ((rc = put_f64(s, dd, d.xwall, FX * 4))) return bail(rc);
+        } else {
+            HIPCHK(hipMemset(dd, 0, FX * 4 * 8));
+        }
+        S.xwall = dd;
+        if ((rc = dev_alloc(s, &p, FY * 4 * 8))) return bail(rc);
+        dd = (double*)p;
+        if (d.ywall) {
+            if ((rc = put_f64(s, dd, d.ywall, FY * 4))) return bail(rc);
+        } else {
+            HIPCHK(hipMemset(dd, 0, FY * 4 * 8));
+        }
+        S.ywall = dd;
+        struct {
+            void** dst;
+            size_t bytes;
+        } bufs[] = {
+            {(void**)&S.w, C * 4 * 8},          {(void**)&S.prim, C * 4 * 8},       {&S.f, C * 2 * d.nv * s->esz},
+            {(void**)&S.sw, C * 8 * 8},         {&S.sf, C * 4 * d.nv * s->esz},     {(void**)&S.xfw, FX * 4 * 8},
+            {&S.xff, FX * 2 * d.nv * s->esz},   {(void**)&S.yfw, FY * 4 * 8},       {&S.yff, FY * 2 * d.nv * s->esz},
+        };
+        for (auto& b : bufs) {
+            if ((rc = dev_alloc(s, &p, b.bytes))) return bail(rc);
+            HIPCHK(hipMemset(p, 0, b.bytes));
+            *b.dst = p;
+        }
+        S.dx = s->dx;
+        S.dy = s->dy;
+        S.u = s->u;
+        S.v = s->v;
+        S.wt = s->wt;
+        if ((rc = dev_alloc(s, &p, C * 16 * 8))) return bail(rc);
+        s->rcell = (double*)p;
+        if ((rc = dev_alloc(s, &p, 8 * 8))) return bail(rc);
+        s->res = (double*)p;
+        if ((rc = dev_alloc(s, &p, sizeof(Solve2D)))) return bail(rc);
+        s->d2 = (Solve2D*)p;
+        if ((rc = push2d(s))) return bail(rc);
+    }
+    *out = s;
+    return KBG_OK;
+}
+
+int kbg_solve_destroy(kbg_solve* s) {
+    if (!s) return KBG_OK;
+    if (s->ctx) (void)hipStreamSynchronize(s->ctx->stream);
+    for (void* p : s->allocs) (void)hipFree(p);
+    delete s;
+    return KBG_OK;
+}
+
+int kbg_upload(kbg_solve* s, const double* w, const double* prim, const double* f) {
+    if (!s || !w || !prim || !f) return fail(KBG_ERR_ARG, "NULL argument");
+    int rc;
+    const size_t N = s->N;
+    if (s->d.dim == 1) {
+        Solve1D& S = s->h1;
+        for (int b = 0; b < 2; b++) {
+            if ((rc = put_f64(s, S.w[b], w, N * s->nw))) return rc;
+            if ((rc = put_vec(s, S.f[b], f, N * s->RS))) return rc;
+        }
+        if ((rc = put_f64(s, S.prim, prim, N * s->nw))) return rc;
+        return KBG_OK;
+    }
+    Solve2D& S = s->h2;
+    if ((rc = put_f64(s, S.w, w, N * 4))) return rc;
+    if ((rc = put_f64(s, S.prim, prim, N * 4))) return rc;
+    return put_vec(s, S.f, f, N * 2 * s->d.nv);
+}
+
+int kbg_download(kbg_solve* s, double* w, double* prim, double* f) {
+    if (!s) return fail(KBG_ERR_ARG, "NULL solve");
+    int rc;
+    const size_t N = s->N;
+    if (s->d.dim == 1) {
+        Solve1D& S = s->h1;
+        if (w && (rc = get_f64(s, w, S.w[S.cur], N * s->nw))) return rc;
+        if (prim && (rc = get_f64(s, prim, S.prim, N * s->nw))) return rc;
+        if (f && (rc = get_vec(s, f, S.f[S.cur], N * s->RS))) return rc;
+        return KBG_OK;
+    }
+    Solve2D& S = s->h2;
+    if (w && (rc = get_f64(s, w, S.w, N * 4))) return rc;
+    if (prim && (rc = get_f64(s, prim, S.prim, N * 4))) return rc;
+    if (f && (rc = get_vec(s, f, S.f, N * 2 * s->d.nv))) return rc;
+    return KBG_OK;
+}
+
+int kbg_upload_slopes(kbg_solve* s, const double* sw, const double* sf) {
+    if (!s) return fail(KBG_ERR_ARG, "NULL solve");
+    int rc;
+    const size_t N = s->N;
+    if (s->d.dim == 1) {
+        if ((rc = ensure_phase_buffers(s))) return rc;
+        if (sw && (rc = put_f64(s, s->h1.sw, sw, N * s->nw))) return rc;
+        if (sf && (rc = put_vec(s, s->h1.sf, sf, N * s->RS))) return rc;
+        return KBG_OK;
+    }
+    if (sw && (rc = put_f64(s, s->h2.sw, sw, N * 8))) return rc;
+    if (sf && (rc = put_vec(s, s->h2.sf, sf, N * 4 * s->d.nv))) return rc;
+    return KBG_OK;
+}
+
+int kbg_download_slopes(kbg_solve* s, double* sw, double* sf) {
+    if (!s) return fail(KBG_ERR_ARG, "NULL solve");
+    int rc;
+    const size_t N = s->N;
+    if (s->d.dim == 1) {
+        if ((rc = ensure_phase_buffers(s))) return rc;
+        if (sw && (rc = get_f64(s, sw, s->h1.sw, N * s->nw))) return rc;
+        if (sf && (rc = get_vec(s, sf, s->h1.sf, N * s->RS))) return rc;
+        return KBG_OK;
+    }
+    if (sw && (rc = get_f64(s, sw, s->h2.sw, N * 8))) return rc;
+    if (sf && (rc = get_vec(s, sf, s->h2.sf, N * 4 * s->d.nv))) return rc;
+    return KBG_OK;
+}
+
+int kbg_upload_faces(kbg_solve* s, const double* fw, const double* ff) {
+    if (!s) return fail(KBG_ERR_ARG, "NULL solve");
+    int rc;
+    if (s->d.dim == 1) {
+        if ((rc = ensure_phase_buffers(s))) return rc;
+        const size_t nf1 = size_t(s->d.nx) + 1;
+        if (fw && (rc = put_f64(s, s->h1.fw, fw, nf1 * s->nw))) return rc;
+        if (ff && (rc = put_vec(s, s->h1.ff, ff, nf1 * s->RS))) return rc;
+        return sync_boundary_faces(s);
+    }
+    const size_t FX = size_t(s->d.nx + 1) * s->d.ny, FY = size_t(s->d.nx) * (s->d.ny + 1), RS = 2 * s->d.nv;
+    if (fw) {
+        if ((rc = put_f64(s, s->h2.xfw, fw, FX * 4))) return rc;
+        if ((rc = put_f64(s, s->h2.yfw, fw + FX * 4, FY * 4))) return rc;
+    }
+    if (ff) {
+        if ((rc = put_vec(s, s->h2.xff, ff, FX * RS))) return rc;
+        if ((rc = put_vec(s, s->h2.yff, ff + FX * RS, FY * RS))) return rc;
+    }
+    return KBG_OK;
+}
+
+int kbg_download_faces(kbg_solve* s, double* fw, double* ff) {
+    if (!s) return fail(KBG_ERR_ARG, "NULL solve");
+    int rc;
+    if (s->d.dim == 1) {
+        if ((rc = ensure_phase_buffers(s))) return rc;
+        const size_t nf1 = size_t(s->d.nx) + 1;
+        if (fw && (rc = get_f64(s, fw, s->h1.fw, nf1 * s->nw))) return rc;
+        if (ff && (rc = get_vec(s, ff, s->h1.ff, nf1 * s->RS))) return rc;
+        return KBG_OK;
+    }
+    const size_t FX = size_t(s->d.nx + 1) * s->d.ny, FY = size_t(s->d.nx) * (s->d.ny + 1), RS = 2 * s->d.nv;
+    if (fw) {
+        if ((rc = get_f64(s, fw, s->h2.xfw, FX * 4))) return rc;
+        if ((rc = get_f64(s, fw + FX * 4, s->h2.yfw, FY * 4))) return rc;
+    }
+    if (ff) {
+        if ((rc = get_vec(s, ff, s->h2.xff, FX * RS))) return rc;
+        if ((rc = get_vec(s, ff + FX * RS, s->h2.yff, FY * RS))) return rc;
+    }
+    return KBG_OK;
+}
+
+int kbg_reconstruct(kbg_solve* s) {
+    if (!s) return fail(KBG_ERR_ARG, "NULL solve");
+    int rc;
+    if (s->d.dim == 1) {
+        if ((rc = ensure_phase_buffers(s)) || (rc = push1d(s))) return rc;
+        HIPCHK(kbg::launch_recon1d(s->fp64, s->h1, s->d1, s->ctx->stream));
+    } else {
+        if ((rc = push2d(s))) return rc;
+        HIPCHK(kbg::launch_recon2d(s->fp64, s->h2, s->d2, s->ctx->stream));
+    }
+    HIPCHK(hipStreamSynchronize(s->ctx->stream));
+    return KBG_OK;
+}
+
+int kbg_evolve(kbg_solve* s, double dt) {
+    if (!s) return fail(KBG_ERR_ARG, "NULL solve");
+    int rc;
+    if (s->d.dim == 1) {
+        if ((rc = ensure_phase_buffers(s))) return rc;
+        s->h1.dt = dt;
+        if ((rc = push1d(s))) return rc;
+        HIPCHK(kbg::launch_evolve1d(s->fp64, s->h1, s->d1, s->ctx->stream));
+        HIPCHK(kbg::launch_wall1d(s->fp64, s->h1, s->d1, 0, 0, s->ctx->stream));
+        if ((rc = sync_boundary_faces(s))) return rc;
+    } else {
+        s->h2.dt = dt;
+        if ((rc = push2d(s))) return rc;
+        HIPCHK(kbg::launch_evolve2d(s->fp64, s->h2, s->d2, s->ctx->stream));
+    }
+    HIPCHK(hipStreamSynchronize(s->ctx->stream));
+    return KBG_OK;
+}
+
+int kbg_update(kbg_solve* s, double dt, double* residual) {
+    if (!s) return fail(KBG_ERR_ARG, "NULL solve");
+    int rc;
+    if (s->d.dim == 1) {
+        if ((rc = ensure_phase_buffers(s))) return rc;
+        s->h1.dt = dt;
+        if ((rc = push1d(s))) return rc;
+        rc = step_phase_1d(s, dt, residual);
+    } else {
+        s->h2.dt = dt;
+        if ((rc = push2d(s))) return rc;
+        rc = step_phase_2d(s, dt, residual);
+    }
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(s->ctx->stream));
+    return KBG_OK;
+}
+
+int kbg_step(kbg_solve* s, double dt, int nsteps, double* residual) {
+    if (!s) return fail(KBG_ERR_ARG, "NULL solve");
+    if (nsteps < 0) return fail(KBG_ERR_ARG, "nsteps < 0");
+    if (nsteps == 0) return KBG_OK;
+    kbg_solve* one[1] = {s};
+    double dts[1] = {dt};
+    return kbg_batch_step(s->ctx, one, 1, dts, nsteps, residual);
+}
+
+int kbg_batch_step(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double* dts, int nsteps, double* residuals) {
+    if (!ctx || (n > 0 && (!solves || !dts))) return fail(KBG_ERR_ARG, "NULL argument");
+    if (nsteps <= 0 || n <= 0) return KBG_OK;
+    hipStream_t st = ctx->stream;
+    int rc;
+    // fused 1D solves grouped by kernel instance; the rest stepped phase by phase
+    std::map<std::tuple<int, int, int, int>, std::vector<int>> groups;
+    std::vector<int> others;
+    for (int i = 0; i < n; i++) {
+        kbg_solve* s = solves[i];
+        if (!s || s->ctx != ctx) return fail(KBG_ERR_ARG, "solve is NULL or belongs to another context");
+        if (s->d.dim == 1 && s->fused) groups[{s->fp64 ? 1 : 0, s->d.nf, s->d.flux, (s->d.nv + 63) / 64}].push_back(i);
+        else others.push_back(i);
+    }
+    // descriptors and work items of all fused solves
+    std::vector<Solve1D> descs;
+    std::vector<int> desc_of(n, -1);
+    struct Group {
+        bool fp64;
+        int nf, flux, vpt;
+        size_t item0, nitems;
+    };
+    std::vector<Group> gl;
+    std::vector<WorkItem> items;
+    const int seg = std::max(1, ctx->seg_cells);
+    for (auto& kv : groups) {
+        Group g{std::get<0>(kv.first) == 1, std::get<1>(kv.first), std::get<2>(kv.first), std::get<3>(kv.first), items.size(), 0};
+        std::vector<WorkItem> gi;
+        for (int i : kv.second) {
+            kbg_solve* s = solves[i];
+            s->h1.dt = dts[i];
+            desc_of[i] = (int)descs.size();
+            build_items(s, seg, desc_of[i], gi);
+            descs.push_back(s->h1);
+        }
+        // longest work first: the dispatcher hands out workgroups in order
+        std::stable_sort(gi.begin(), gi.end(), [](const WorkItem& a, const WorkItem& b) {
+            return (long)(a.c1 - a.c0 + 1) * a.vpt > (long)(b.c1 - b.c0 + 1) * b.vpt;
+        });
+        g.nitems = gi.size();
+        items.insert(items.end(), gi.begin(), gi.end());
+        gl.push_back(g);
+    }
+    if (!descs.empty()) {
+        if (descs.size() > ctx->descs_cap) {
+            (void)hipFree(ctx->d_descs);
+            ctx->descs_cap = descs.size() * 2;
+            HIPCHK(hipMalloc((void**)&ctx->d_descs, ctx->descs_cap * sizeof(Solve1D)));
+        }
+        if (items.size() > ctx->items_cap) {
+            (void)hipFree(ctx->d_items);
+            (void)hipFree(ctx->d_rpart);
+            ctx->items_cap = items.size() * 2;
+            HIPCHK(hipMalloc((void**)&ctx->d_items, ctx->items_cap * sizeof(WorkItem)));
+            HIPCHK(hipMalloc((void**)&ctx->d_rpart, ctx->items_cap * 12 * sizeof(double)));
+        }
+        HIPCHK(hipMemcpyAsync(ctx->d_descs, descs.data(), descs.size() * sizeof(Solve1D), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(ctx->d_items, items.data(), items.size() * sizeof(WorkItem), hipMemcpyHostToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));
+    }
+    for (int t = 0; t < nsteps; t++) {
+        const int par = t & 1;
+        const bool last = t == nsteps - 1;
+        for (int i = 0; i < n; i++) {
+            if (desc_of[i] < 0) continue;
+            const Solve1D& S = descs[desc_of[i]];
+            if (S.wallL || S.wallR) HIPCHK(kbg::launch_wall1d(solves[i]->fp64, S, ctx->d_descs + desc_of[i], 1, par, st));
+        }
+        for (const Group& g : gl) {
+            double* rp = (last && residuals) ? ctx->d_rpart + g.item0 * 12 : nullptr;
+            HIPCHK(kbg::launch_march1d(g.fp64, g.nf, g.flux, g.vpt, ctx->d_descs, ctx->d_items + g.item0, (int)g.nitems,
+                                       grid_for(ctx, (int)g.nitems), par, rp, seg, st));
+        }
+        for (int i : others) {
+            kbg_solve* s = solves[i];
+            double* r = (last && residuals) ? residuals + 5 * (size_t)i : nullptr;
+            if (s->d.dim == 1) {
+                if ((rc = ensure_phase_buffers(s))) return rc;
+                if (t == 0) {
+                    s->h1.dt = dts[i];
+                    if ((rc = push1d(s))) return rc;
+                }
+                HIPCHK(kbg::launch_recon1d(s->fp64, s->h1, s->d1, st));
+                HIPCHK(kbg::launch_evolve1d(s->fp64, s->h1, s->d1, st));
+                HIPCHK(kbg::launch_wall1d(s->fp64, s->h1, s->d1, 0, 0, st));
+                if ((rc = step_phase_1d(s, dts[i], r))) return rc;
+            } else {
+                if (t == 0) {
+                    s->h2.dt = dts[i];
+                    if ((rc = push2d(s))) return rc;
+                }
+                HIPCHK(kbg::launch_recon2d(s->fp64, s->h2, s->d2, st));
+                HIPCHK(kbg::launch_evolve2d(s->fp64, s->h2, s->d2, st));
+                if ((rc = step_phase_2d(s, dts[i], r))) return rc;
+            }
+        }
+    }
+    // residuals of the fused solves: per-item partial rows reduced per solve
+    if (residuals) {
+        std::vector<int> fused_idx;
+        for (int i = 0; i < n; i++)
+            if (desc_of[i] >= 0) fused_idx.push_back(i);
+        if (!fused_idx.empty()) {
+            std::vector<double> rp(items.size() * 12);
+            HIPCHK(hipMemcpyAsync(rp.data(), ctx->d_rpart, rp.size() * 8, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            std::vector<double> acc(descs.size() * 12, 0.0);
+            for (size_t it = 0; it < items.size(); it++)
+                for (int m = 0; m < 12; m++) acc[items[it].solve * 12 + m] += rp[it * 12 + m];
+            for (int i : fused_idx) {
+                const double* a = acc.data() + desc_of[i] * 12;
+                const double nx = solves[i]->d.nx;
+                for (int m = 0; m < 3; m++)
+                    residuals[5 * (size_t)i + m] = std::sqrt(a[m] * nx) / (a[3 + m] + 1.e-7) + std::sqrt(a[6 + m] * 2.0) / (a[9 + m] + 1.e-7);
+            }
+        }
+    }
+    for (int i = 0; i < n; i++) {
+        if (desc_of[i] < 0) continue;
+        kbg_solve* s = solves[i];
+        s->h1.cur = (s->h1.cur + nsteps) & 1;
+        if ((rc = push1d(s))) return rc;
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    return KBG_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// update! on the phase-exact path (kernels k_update1d / k_resid), optional residual download
+int step_phase_1d(kbg_solve* s, double dt, double* residual) {
+    (void)dt;
+    hipStream_t st = s->ctx->stream;
+    double* rc_ = residual ? s->rcell : nullptr;
+    HIPCHK(kbg::launch_update1d(s->fp64, s->h1, s->d1, rc_, st));
+    if (residual) {
+        const int nrows = s->h1.upd_hi - s->h1.upd_lo + 1;
+        HIPCHK(kbg::launch_resid1d(s->h1, s->rcell, nrows, 4 * s->nw, s->res, st));
+        HIPCHK(hipMemcpyAsync(residual, s->res, s->nw * 8, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+    }
+    return KBG_OK;
+}
+
+int step_phase_2d(kbg_solve* s, double dt, double* residual) {
+    (void)dt;
+    hipStream_t st = s->ctx->stream;
+    double* rc_ = residual ? s->rcell : nullptr;
+    HIPCHK(kbg::launch_update2d(s->fp64, s->h2, s->d2, rc_, st));
+    if (residual) {
+        HIPCHK(kbg::launch_resid2d(s->h2, s->rcell, s->res, st));
+        HIPCHK(hipMemcpyAsync(residual, s->res, 4 * 8, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+    }
+    return KBG_OK;
+}
+
+}  // namespace

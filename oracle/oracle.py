@@ -1,0 +1,166 @@
+"""ctypes binding of the FP64 CPU oracle (oracle/liboracle.so, built from kitbase_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: the checker for the HIP path.  Only tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg may import this module.  The oracle restates KitBase.jl v0.9.31 (Julia,
+which is not installed here and cannot be built or run), function by function; the citations are
+in kitbase_oracle.h / kitbase_oracle.c.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+import sys
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+
+_dp = C.POINTER(C.c_double)
+_lib = None
+
+
+def build() -> Path:
+    subprocess.run(["make", "-C", str(HERE), "-s"], check=True)
+    return LIB
+
+
+def load() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        _lib = C.CDLL(str(LIB))
+        for name in ("orc_heaviside", "orc_fortsign", "orc_newton_cotes", "orc_ref_vhs_vis",
+                     "orc_vhs_collision_time", "orc_heat_capacity_ratio", "orc_sound_speed"):
+            getattr(_lib, name).restype = C.c_double
+        _lib.orc_limiter.restype = C.c_double
+        _lib.orc_limiter.argtypes = [C.c_int, C.c_double, C.c_double]
+        _lib.orc_heaviside.argtypes = [C.c_double]
+        _lib.orc_fortsign.argtypes = [C.c_double, C.c_double]
+        _lib.orc_newton_cotes.argtypes = [C.c_int, C.c_int]
+        _lib.orc_ref_vhs_vis.argtypes = [C.c_double] * 3
+        _lib.orc_vhs_collision_time.argtypes = [C.c_double] * 4
+    return _lib
+
+
+def p(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"], "oracle arrays must be C-contiguous float64"
+    return a.ctypes.data_as(_dp)
+
+
+def vec(x, n=None):
+    a = np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+    return a
+
+
+class Solver1DStruct(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int), ("ng", C.c_int), ("nf", C.c_int), ("vdim", C.c_int), ("nv", C.c_int),
+        ("flux", C.c_int), ("coll", C.c_int), ("limiter", C.c_int), ("interp_order", C.c_int),
+        ("bc", C.c_int * 2),
+        ("K", C.c_double), ("gam", C.c_double), ("mu", C.c_double), ("om", C.c_double), ("Pr", C.c_double),
+        ("u", _dp), ("v", _dp), ("wv", _dp), ("weights", _dp), ("dx", _dp), ("bc_prim", _dp * 2),
+        ("w", _dp), ("prim", _dp), ("f", _dp), ("sw", _dp), ("sf", _dp), ("fw", _dp), ("ff", _dp),
+    ]
+
+
+class Solver2DStruct(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("nv", C.c_int),
+        ("flux", C.c_int), ("coll", C.c_int), ("limiter", C.c_int), ("interp_order", C.c_int),
+        ("bc", C.c_int * 4),
+        ("K", C.c_double), ("gam", C.c_double), ("mu", C.c_double), ("om", C.c_double), ("Pr", C.c_double),
+        ("u", _dp), ("v", _dp), ("weights", _dp), ("dx", _dp), ("dy", _dp),
+        ("xn", _dp), ("xlen", _dp), ("xwall", _dp), ("yn", _dp), ("ylen", _dp), ("ywall", _dp),
+        ("w", _dp), ("prim", _dp), ("f", _dp), ("sw", _dp), ("sf", _dp),
+        ("xfw", _dp), ("xff", _dp), ("yfw", _dp), ("yff", _dp),
+    ]
+
+
+FLUX = {"kfvs": 0, "kcu": 1, "ugks": 2}
+COLL = {"bgk": 0, "shakhov": 1}
+BC = {"fix": 0, "maxwell": 1}
+LIM = {"vanleer": 0, "minmod": 1, "superbee": 2, "vanalbaba": 3, "linear": 4}
+
+
+class OracleSolve:
+    """The oracle's view of a SolverSet plus host arrays (modified in place)."""
+
+    def __init__(self, ks, ctr, face):
+        self.lib = load()
+        self.ks, self.ctr, self.face = ks, ctr, face
+        self.keep = []
+        g = ks.gas
+        bcs = ks.boundaries()
+
+        def k(a):
+            a = np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+            self.keep.append(a)
+            return p(a)
+
+        if ks.dim == 1:
+            S = Solver1DStruct()
+            ps, vs = ks.ps, ks.vs
+            S.nx, S.ng, S.nf, S.vdim, S.nv = ps.nx, ps.ng, ks.nf, ks.vdim, vs.nv
+            S.flux, S.coll, S.limiter, S.interp_order = FLUX[ks.set.flux], COLL[ks.set.collision], LIM[ks.set.limiter], ks.set.interpOrder
+            S.bc[0], S.bc[1] = BC[bcs[0]], BC[bcs[1]]
+            S.K, S.gam, S.mu, S.om, S.Pr = g.K, g.gamma, g.mu_ref, g.omega, g.Pr
+            S.u, S.weights = k(vs.u), k(vs.weights)
+            if ks.vdim == 3:
+                S.v, S.wv = k(vs.v), k(vs.w)
+            S.dx = k(ps.dx)
+            S.bc_prim[0], S.bc_prim[1] = k(ks.ib.bc(ps.x0)), k(ks.ib.bc(ps.x1))
+            S.w, S.prim, S.f, S.sw, S.sf = p(ctr.w), p(ctr.prim), p(ctr.f), p(ctr.sw), p(ctr.sf)
+            S.fw, S.ff = p(face.fw), p(face.ff)
+        else:
+            kb = sys.modules["kitbase_amd"]  # the host mirror supplies the face geometry
+            S = Solver2DStruct()
+            ps, vs = ks.ps, ks.vs
+            S.nx, S.ny, S.nv = ps.nx, ps.ny, vs.nv
+            S.flux, S.coll, S.limiter, S.interp_order = FLUX[ks.set.flux], COLL[ks.set.collision], LIM[ks.set.limiter], ks.set.interpOrder
+            for i in range(4):
+                S.bc[i] = BC[bcs[i]]
+            S.K, S.gam, S.mu, S.om, S.Pr = g.K, g.gamma, g.mu_ref, g.omega, g.Pr
+            S.u, S.v, S.weights = k(vs.u), k(vs.v), k(vs.weights)
+            S.dx, S.dy = k(ps.dx), k(ps.dy)
+            xg, yg = kb.geometry.face_geometry_2d(ps)
+            xw, yw = kb.solver.wall_prims_2d(ks)
+            S.xn, S.xlen, S.xwall = k(xg[..., 0:2]), k(xg[..., 2]), k(xw)
+            S.yn, S.ylen, S.ywall = k(yg[..., 0:2]), k(yg[..., 2]), k(yw)
+            S.w, S.prim, S.f, S.sw, S.sf = p(ctr.w), p(ctr.prim), p(ctr.f), p(ctr.sw), p(ctr.sf)
+            S.xfw, S.xff, S.yfw, S.yff = p(face.fw), p(face.ff), p(face.yfw), p(face.yff)
+        self.S = S
+        self.dim = ks.dim
+
+    def _call(self, name, *args):
+        rc = getattr(self.lib, f"orc_{name}_{self.dim}d")(C.byref(self.S), *args)
+        if rc != 0:
+            raise RuntimeError(f"oracle {name} returned {rc}")
+
+    def reconstruct(self):
+        self._call("reconstruct")
+
+    def evolve(self, dt):
+        self._call("evolve", C.c_double(dt))
+
+    def update(self, dt, res=None):
+        r = np.zeros(5) if res is None else res
+        self._call("update", C.c_double(dt), p(r))
+        return r
+
+    def run(self, dt, nsteps, res=None):
+        r = np.zeros(5) if res is None else res
+        self._call("run", C.c_double(dt), C.c_int(nsteps), p(r))
+        return r
+
+
+def set_threads(n: int):
+    load().orc_set_num_threads(int(n))
+
+
+def num_threads() -> int:
+    return int(load().orc_num_threads())

@@ -1,0 +1,279 @@
+"""GPU parity: the HIP path (through the C ABI) against the FP64 oracle on the same inputs.
+
+Tolerances: FP64 storage <= 1e-6 relative (observed ~1e-12); FP32 storage <= 1e-4 relative L2 in
+density, velocity and temperature (the `north_star` bar).
+"""
+import copy
+
+import numpy as np
+import pytest
+
+from conftest import kb, oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    a, b = np.asarray(a).ravel(), np.asarray(b).ravel()
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def rel_max(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def macro(prim):
+    """density, velocity (all components), temperature T = 1/(2 lambda)"""
+    p = np.asarray(prim)
+    return p[..., 0], p[..., 1:-1], 1.0 / (2.0 * p[..., -1])
+
+
+def check_macro(pg, po, tol):
+    rg, ug, tg = macro(pg)
+    ro, uo, to = macro(po)
+    assert rel_l2(rg, ro) <= tol, ("rho", rel_l2(rg, ro))
+    assert rel_l2(tg, to) <= tol, ("T", rel_l2(tg, to))
+    # velocity: relative to the velocity scale (it crosses zero)
+    scale = max(np.linalg.norm(uo), 1e-12 * uo.size)
+    assert float(np.linalg.norm(ug - uo) / scale) <= tol, ("U", float(np.linalg.norm(ug - uo) / scale))
+
+
+def run_both(cfg, nsteps, precision, fused=True, ctx=None, prepare=None):
+    ks = kb.SolverSet(cfg)
+    ctr, face = kb.init_fvm(ks)
+    if prepare:
+        prepare(ks, ctr, face)
+    dt = kb.timestep(ks, ctr)
+    c_o, f_o = copy.deepcopy(ctr), copy.deepcopy(face)
+    o = oracle.OracleSolve(ks, c_o, f_o)
+    o.run(dt, nsteps)
+    dev = kb.DeviceSolve(ks, ctr, face, precision=precision, ctx=ctx)
+    if fused:
+        dev.step(dt, nsteps)
+    else:
+        for _ in range(nsteps):
+            dev.reconstruct()
+            dev.evolve(dt)
+            dev.update(dt)
+    c_g = copy.deepcopy(ctr)
+    dev.download(c_g)
+    dev.close()
+    return ks, c_g, c_o
+
+
+# ---------------------------------------------------------------- 1D1F1V (Sod)
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-6), ("fp32", 1e-4)])
+def test_sod_fused_full_run(gpu_ctx, precision, tol):
+    """Config 1 end to end: 100 x 64 Newton-Cotes, KFVS + BGK, 270 steps to t = 0.2."""
+    ks, cg, co = run_both(kb.configs.sod_1d1f1v(), 270, precision, ctx=gpu_ctx)
+    check_macro(cg.prim, co.prim, tol)
+    if precision == "fp64":
+        assert rel_max(cg.f, co.f) <= 1e-9
+        assert rel_max(cg.w, co.w) <= 1e-9
+
+
+@pytest.mark.parametrize("coll", ["bgk", "shakhov"])
+@pytest.mark.parametrize("flux", ["kfvs", "kcu"])
+@pytest.mark.parametrize("ng", [0, 2])
+def test_1d1f1v_variants_fp64(gpu_ctx, coll, flux, ng):
+    cfg = kb.configs.sod_1d1f1v(collision=coll, flux=flux, nxg=ng, nx=60, knudsen=1e-2, prandtl=2.0 / 3.0)
+    ks, cg, co = run_both(cfg, 40, "fp64", ctx=gpu_ctx)
+    assert rel_max(cg.f, co.f) <= 1e-9, rel_max(cg.f, co.f)
+    assert rel_max(cg.w, co.w) <= 1e-9
+
+
+@pytest.mark.parametrize("limiter", ["minmod", "superbee", "vanalbaba", "linear"])
+def test_1d_limiters_fp64(gpu_ctx, limiter):
+    cfg = kb.configs.sod_1d1f1v(limiter=limiter, nx=50)
+    ks, cg, co = run_both(cfg, 20, "fp64", ctx=gpu_ctx)
+    assert rel_max(cg.f, co.f) <= 1e-9
+
+
+def test_1d_first_order_fp64(gpu_ctx):
+    ks, cg, co = run_both(kb.configs.sod_1d1f1v(interpOrder=1, nx=50), 20, "fp64", ctx=gpu_ctx)
+    assert rel_max(cg.f, co.f) <= 1e-9
+
+
+@pytest.mark.parametrize("seg", [1, 3, 7, 64, 252])
+def test_fused_segmentation_invariance(gpu_ctx, seg):
+    """Work-item length must not change results (halo rows, carried faces, block tails)."""
+    old = gpu_ctx.schedule()
+    try:
+        gpu_ctx.set_schedule(seg, 0)
+        ks, cg, co = run_both(kb.configs.sod_1d1f1v(collision="shakhov", nx=70, knudsen=1e-2), 15, "fp64", ctx=gpu_ctx)
+        assert rel_max(cg.f, co.f) <= 1e-9
+    finally:
+        gpu_ctx.set_schedule(*old)
+
+
+@pytest.mark.parametrize("nu", [32, 64, 65, 128, 150, 256, 300])
+def test_velocity_counts_fp32(gpu_ctx, nu):
+    """Every lane mapping (VPT 1..4) and the phase-exact fallback above 256 velocities."""
+    cfg = kb.configs.sod_1d1f1v(nu=nu, vMeshType="rectangle", nx=80, collision="shakhov", knudsen=1e-3)
+    ks, cg, co = run_both(cfg, 30, "fp32", ctx=gpu_ctx)
+    check_macro(cg.prim, co.prim, 1e-4)
+
+
+def test_phase_exact_1d_matches_oracle_phases(gpu_ctx):
+    """reconstruct!, evolve!, update! individually: slopes, face fluxes and the stepped cells."""
+    ks = kb.SolverSet(kb.configs.sod_1d1f1v(nx=40, collision="shakhov", knudsen=1e-2))
+    ctr, face = kb.init_fvm(ks)
+    dt = kb.timestep(ks, ctr)
+    co, fo = copy.deepcopy(ctr), copy.deepcopy(face)
+    o = oracle.OracleSolve(ks, co, fo)
+    dev = kb.DeviceSolve(ks, ctr, face, precision="fp64", ctx=gpu_ctx)
+    cg, fg = copy.deepcopy(ctr), copy.deepcopy(face)
+    for it in range(3):
+        o.reconstruct()
+        dev.reconstruct()
+        dev.download(cg, slopes=True)
+        assert rel_max(cg.sf, co.sf) <= 1e-12 and rel_max(cg.sw, co.sw) <= 1e-12
+        o.evolve(dt)
+        dev.evolve(dt)
+        dev.download(cg, fg)
+        assert rel_max(fg.ff, fo.ff) <= 1e-12
+        assert rel_max(fg.fw, fo.fw) <= 1e-11
+        ro = o.update(dt)
+        rg = np.zeros(3)
+        dev.update(dt, rg)
+        dev.download(cg)
+        assert rel_max(cg.f, co.f) <= 1e-12
+        assert rel_max(cg.w, co.w) <= 1e-12
+        assert np.allclose(rg, ro[:3], rtol=1e-8)
+    dev.close()
+
+
+def test_fused_and_phase_paths_agree_fp32(gpu_ctx):
+    cfg = kb.configs.sod_1d1f1v(nx=90, collision="shakhov", knudsen=1e-3)
+    _, c1, co = run_both(cfg, 25, "fp32", fused=True, ctx=gpu_ctx)
+    _, c2, _ = run_both(cfg, 25, "fp32", fused=False, ctx=gpu_ctx)
+    check_macro(c1.prim, c2.prim, 1e-5)
+
+
+# ---------------------------------------------------------------- 1D2F1V (shock structure)
+
+@pytest.mark.parametrize("flux", ["kfvs", "kcu", "ugks"])
+@pytest.mark.parametrize("coll", ["bgk", "shakhov"])
+def test_1d2f1v_fp64(gpu_ctx, flux, coll):
+    cfg = kb.configs.shock_1d2f1v(flux=flux, collision=coll, nx=80, nu=48)
+    ks, cg, co = run_both(cfg, 30, "fp64", ctx=gpu_ctx)
+    assert rel_max(cg.f, co.f) <= 1e-9, rel_max(cg.f, co.f)
+    assert rel_max(cg.w, co.w) <= 1e-9
+
+
+def test_1d2f1v_maxwell_walls_fp64(gpu_ctx):
+    cfg = kb.configs.shock_1d2f1v(boundary="maxwell", nx=60, nu=40, flux="kfvs")
+    for fused in (True, False):
+        ks, cg, co = run_both(cfg, 20, "fp64", fused=fused, ctx=gpu_ctx)
+        assert rel_max(cg.f, co.f) <= 1e-9
+        assert rel_max(cg.w, co.w) <= 1e-9
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-6), ("fp32", 1e-4)])
+def test_shock_config_ugks_shakhov(gpu_ctx, precision, tol):
+    """Config 2 (400 x 101, UGKS + Shakhov) over 300 of its 2000 steps."""
+    ks, cg, co = run_both(kb.configs.shock_1d2f1v(), 300, precision, ctx=gpu_ctx)
+    check_macro(cg.prim, co.prim, tol)
+
+
+# ---------------------------------------------------------------- 1D1F3V
+
+def test_1d1f3v_fp64(gpu_ctx):
+    cfg = kb.configs.shock_1d1f3v(nx=24, nu=8, nv=8, nw=8)
+    for flux in ("kfvs", "kcu"):
+        cfg["flux"] = flux
+        ks, cg, co = run_both(cfg, 10, "fp64", ctx=gpu_ctx)
+        assert rel_max(cg.f, co.f) <= 1e-9
+        assert rel_max(cg.w, co.w) <= 1e-9
+
+
+# ---------------------------------------------------------------- 2D2F2V cavity
+
+@pytest.mark.parametrize("flux", ["kfvs", "kcu", "ugks"])
+@pytest.mark.parametrize("coll", ["bgk", "shakhov"])
+def test_cavity_small_fp64(gpu_ctx, flux, coll):
+    cfg = kb.configs.cavity_2d2f2v(flux=flux, collision=coll, nx=12, ny=10, nu=12, nv=12)
+    ks, cg, co = run_both(cfg, 8, "fp64", ctx=gpu_ctx)
+    assert rel_max(cg.f, co.f) <= 1e-9, rel_max(cg.f, co.f)
+    assert rel_max(cg.w, co.w) <= 1e-9
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-6), ("fp32", 1e-4)])
+def test_cavity_config_ugks(gpu_ctx, precision, tol):
+    """Config 3 geometry (64 x 64, 32 x 32 Maxwell quadrature, UGKS) over 40 steps."""
+    ks, cg, co = run_both(kb.configs.cavity_2d2f2v(), 40, precision, ctx=gpu_ctx)
+    check_macro(cg.prim, co.prim, tol)
+
+
+def test_cavity_phase_exact(gpu_ctx):
+    ks = kb.SolverSet(kb.configs.cavity_2d2f2v(nx=8, ny=9, nu=10, nv=10, flux="ugks", collision="shakhov"))
+    ctr, face = kb.init_fvm(ks)
+    dt = kb.timestep(ks, ctr)
+    co, fo = copy.deepcopy(ctr), copy.deepcopy(face)
+    o = oracle.OracleSolve(ks, co, fo)
+    dev = kb.DeviceSolve(ks, ctr, face, precision="fp64", ctx=gpu_ctx)
+    cg, fg = copy.deepcopy(ctr), copy.deepcopy(face)
+    for it in range(2):
+        o.reconstruct()
+        dev.reconstruct()
+        o.evolve(dt)
+        dev.evolve(dt)
+        dev.download(cg, fg, slopes=True)
+        assert rel_max(cg.sf, co.sf) <= 1e-12
+        assert rel_max(fg.ff, fo.ff) <= 1e-11 and rel_max(fg.yff, fo.yff) <= 1e-11
+        assert rel_max(fg.fw, fo.fw) <= 1e-10 and rel_max(fg.yfw, fo.yfw) <= 1e-10
+        ro = o.update(dt)
+        rg = np.zeros(4)
+        dev.update(dt, rg)
+        dev.download(cg)
+        assert rel_max(cg.f, co.f) <= 1e-11
+        assert np.allclose(rg, ro[:4], rtol=1e-8)
+    dev.close()
+
+
+# ---------------------------------------------------------------- batching
+
+def _instance(inst, precision, ctx):
+    cfg = kb.configs.riemann_1d1f1v(inst)
+    ks = kb.SolverSet(cfg)
+    ctr, face = kb.init_fvm(ks)
+    dt = kb.timestep(ks, ctr)
+    return ks, ctr, face, dt
+
+
+def test_batch_equals_individual_and_oracle(gpu_ctx):
+    insts = kb.configs.batch_instances(12, seed=7)
+    for i in insts:  # keep the oracle side quick
+        i["nx"] = min(i["nx"], 160)
+    steps = 12
+    solo, batch, dts, oracles = [], [], [], []
+    for inst in insts:
+        ks, ctr, face, dt = _instance(inst, "fp32", gpu_ctx)
+        solo.append((ks, copy.deepcopy(ctr), kb.DeviceSolve(ks, ctr, face, "fp32", gpu_ctx)))
+        batch.append(kb.DeviceSolve(ks, ctr, face, "fp32", gpu_ctx))
+        co, fo = copy.deepcopy(ctr), copy.deepcopy(face)
+        oracle.OracleSolve(ks, co, fo).run(dt, steps)
+        oracles.append(co)
+        dts.append(dt)
+    kb.batch_step(batch, dts, steps, ctx=gpu_ctx)
+    for (ks, c, d), b, dt, co in zip(solo, batch, dts, oracles):
+        d.step(dt, steps)
+        c1, c2 = copy.deepcopy(c), copy.deepcopy(c)
+        d.download(c1)
+        b.download(c2)
+        assert np.array_equal(c1.f, c2.f), "batched and single-solve launches must agree bit for bit"
+        check_macro(c2.prim, co.prim, 1e-4)
+
+
+def test_residual_fused_matches_oracle(gpu_ctx):
+    ks = kb.SolverSet(kb.configs.sod_1d1f1v(nx=64))
+    ctr, face = kb.init_fvm(ks)
+    dt = kb.timestep(ks, ctr)
+    co, fo = copy.deepcopy(ctr), copy.deepcopy(face)
+    ro = oracle.OracleSolve(ks, co, fo).run(dt, 5)
+    dev = kb.DeviceSolve(ks, ctr, face, "fp64", gpu_ctx)
+    rg = np.zeros(3)
+    dev.step(dt, 5, rg)
+    assert np.allclose(rg, ro[:3], rtol=1e-8)
