@@ -87,7 +87,11 @@ int kbg_ctx_sync(kbg_ctx *ctx);
 /* Fused-kernel schedule: cells per work item and wavefronts launched (0 = one per item). */
 int kbg_ctx_set_schedule(kbg_ctx *ctx, int seg_cells, int waves);
 int kbg_ctx_get_schedule(kbg_ctx *ctx, int *seg_cells, int *waves);
-/* HIP events on the context stream, for timing: record slot (0..15), elapsed ms between slots. */
+/* HIP events on the context stream, for timing: record slot (0..15), elapsed ms between slots.
+ * kbg_step / kbg_batch_step record slots KBG_EV_STEPS_BEGIN and KBG_EV_STEPS_END around their
+ * time-step loop (after the descriptor upload, before the residual download), so the elapsed
+ * time between them is the device time of the nsteps steps alone. */
+enum { KBG_EV_STEPS_BEGIN = 14, KBG_EV_STEPS_END = 15 };
 int kbg_ctx_event_record(kbg_ctx *ctx, int slot);
 int kbg_ctx_event_elapsed(kbg_ctx *ctx, int slot_a, int slot_b, float *ms);
 

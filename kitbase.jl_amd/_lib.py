@@ -19,6 +19,7 @@ KBG_KFVS, KBG_KCU, KBG_UGKS = 0, 1, 2
 KBG_BGK, KBG_SHAKHOV = 0, 1
 KBG_BC_FIX, KBG_BC_MAXWELL = 0, 1
 KBG_VANLEER, KBG_MINMOD, KBG_SUPERBEE, KBG_VANALBABA, KBG_LINEAR = 0, 1, 2, 3, 4
+KBG_EV_STEPS_BEGIN, KBG_EV_STEPS_END = 14, 15  # event slots around kbg_step / kbg_batch_step loops
 
 FLUX = {"kfvs": KBG_KFVS, "kcu": KBG_KCU, "ugks": KBG_UGKS}
 COLL = {"bgk": KBG_BGK, "shakhov": KBG_SHAKHOV}

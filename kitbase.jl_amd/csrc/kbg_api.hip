@@ -43,6 +43,11 @@ struct kbg_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[16] = {};
+    // independent kernel groups of a batch run concurrently: group 0 on `stream`, the others on
+    // these side streams (default priority), forked from and joined back into `stream`
+    static constexpr int kSide = 3;
+    hipStream_t side[kSide] = {};
+    hipEvent_t fork = nullptr, join[kSide] = {};
     int seg_cells = 64;
     int waves = 0;
     // batch staging (device)
@@ -325,25 +330,39 @@ int kbg_ctx_create(int device, kbg_ctx** out) {
     kbg_ctx* c = new kbg_ctx();
     c->device = device;
     e = hipStreamCreate(&c->stream);
+    for (auto& sd : c->side)
+        if (e == hipSuccess) e = hipStreamCreate(&sd);
+    for (auto& ev : c->ev)
+        if (e == hipSuccess) e = hipEventCreate(&ev);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->fork, hipEventDisableTiming);
+    for (auto& ev : c->join)
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipMalloc((void**)&c->d_res, 4096 * 5 * sizeof(double));
     if (e != hipSuccess) {
-        delete c;
-        return fail(KBG_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+        kbg_ctx_destroy(c);
+        return fail(KBG_ERR_HIP, std::string("kbg_ctx_create: ") + hipGetErrorString(e));
     }
-    for (auto& ev : c->ev) HIPCHK(hipEventCreate(&ev));
-    HIPCHK(hipMalloc((void**)&c->d_res, 4096 * 5 * sizeof(double)));
     *out = c;
     return KBG_OK;
 }
 
 int kbg_ctx_destroy(kbg_ctx* c) {
     if (!c) return KBG_OK;
-    (void)hipStreamSynchronize(c->stream);
-    for (auto& ev : c->ev) (void)hipEventDestroy(ev);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto& sd : c->side)
+        if (sd) (void)hipStreamSynchronize(sd);
+    for (auto& ev : c->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    for (auto& ev : c->join)
+        if (ev) (void)hipEventDestroy(ev);
+    if (c->fork) (void)hipEventDestroy(c->fork);
     (void)hipFree(c->d_descs);
     (void)hipFree(c->d_items);
     (void)hipFree(c->d_rpart);
     (void)hipFree(c->d_res);
-    (void)hipStreamDestroy(c->stream);
+    for (auto& sd : c->side)
+        if (sd) (void)hipStreamDestroy(sd);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return KBG_OK;
 }
@@ -730,12 +749,14 @@ int kbg_batch_step(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double* 
         bool fp64;
         int nf, flux, vpt;
         size_t item0, nitems;
+        std::vector<int> members;  // solve indices
     };
     std::vector<Group> gl;
     std::vector<WorkItem> items;
     const int seg = std::max(1, ctx->seg_cells);
     for (auto& kv : groups) {
-        Group g{std::get<0>(kv.first) == 1, std::get<1>(kv.first), std::get<2>(kv.first), std::get<3>(kv.first), items.size(), 0};
+        Group g{std::get<0>(kv.first) == 1, std::get<1>(kv.first), std::get<2>(kv.first), std::get<3>(kv.first), items.size(), 0,
+                kv.second};
         std::vector<WorkItem> gi;
         for (int i : kv.second) {
             kbg_solve* s = solves[i];
@@ -769,18 +790,29 @@ int kbg_batch_step(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double* 
         HIPCHK(hipMemcpyAsync(ctx->d_items, items.data(), items.size() * sizeof(WorkItem), hipMemcpyHostToDevice, st));
         HIPCHK(hipStreamSynchronize(st));
     }
+    HIPCHK(hipEventRecord(ctx->ev[KBG_EV_STEPS_BEGIN], st));
+    // group gi runs on stream gst[gi]; groups share no data, so they only fork from and join
+    // back into the context stream
+    const int nside = std::min<int>(kbg_ctx::kSide, (int)gl.size() - 1);
+    std::vector<hipStream_t> gst(gl.size(), st);
+    if (nside > 0) {
+        HIPCHK(hipEventRecord(ctx->fork, st));
+        for (int k = 0; k < nside; k++) HIPCHK(hipStreamWaitEvent(ctx->side[k], ctx->fork, 0));
+        for (size_t gi = 1; gi < gl.size(); gi++) gst[gi] = ctx->side[(gi - 1) % kbg_ctx::kSide];
+    }
     for (int t = 0; t < nsteps; t++) {
         const int par = t & 1;
         const bool last = t == nsteps - 1;
-        for (int i = 0; i < n; i++) {
-            if (desc_of[i] < 0) continue;
-            const Solve1D& S = descs[desc_of[i]];
-            if (S.wallL || S.wallR) HIPCHK(kbg::launch_wall1d(solves[i]->fp64, S, ctx->d_descs + desc_of[i], 1, par, st));
-        }
-        for (const Group& g : gl) {
+        for (size_t gi = 0; gi < gl.size(); gi++) {
+            const Group& g = gl[gi];
+            for (int di : g.members) {
+                const Solve1D& S = descs[desc_of[di]];
+                if (S.wallL || S.wallR)
+                    HIPCHK(kbg::launch_wall1d(solves[di]->fp64, S, ctx->d_descs + desc_of[di], 1, par, gst[gi]));
+            }
             double* rp = (last && residuals) ? ctx->d_rpart + g.item0 * 12 : nullptr;
             HIPCHK(kbg::launch_march1d(g.fp64, g.nf, g.flux, g.vpt, ctx->d_descs, ctx->d_items + g.item0, (int)g.nitems,
-                                       grid_for(ctx, (int)g.nitems), par, rp, seg, st));
+                                       grid_for(ctx, (int)g.nitems), par, rp, seg, gst[gi]));
         }
         for (int i : others) {
             kbg_solve* s = solves[i];
@@ -806,6 +838,11 @@ int kbg_batch_step(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double* 
             }
         }
     }
+    for (int k = 0; k < nside; k++) {
+        HIPCHK(hipEventRecord(ctx->join[k], ctx->side[k]));
+        HIPCHK(hipStreamWaitEvent(st, ctx->join[k], 0));
+    }
+    HIPCHK(hipEventRecord(ctx->ev[KBG_EV_STEPS_END], st));
     // residuals of the fused solves: per-item partial rows reduced per solve
     if (residuals) {
         std::vector<int> fused_idx;

@@ -316,8 +316,14 @@ def init_fvm(ks: SolverSet) -> tuple[ControlVolumes, Interfaces]:
         N, nw, nf = ps.x.size, ks.nw, ks.nf
         w = np.empty((N, nw))
         f = np.empty((N, nf, nv))
+        seen: dict[bytes, int] = {}  # piecewise-constant initial states: evaluate each once
         for k, x in enumerate(ps.x):
             w[k] = ks.ib.fw(x)
+            key = w[k].tobytes()
+            if key in seen:
+                f[k] = f[seen[key]]
+                continue
+            seen[key] = k
             ff = ks.ib.ff(x)
             if nf == 1:
                 f[k, 0] = ff
@@ -422,6 +428,10 @@ class Context:
         ms = C.c_float()
         _lib.check(self.lib.kbg_ctx_event_elapsed(self.handle, a, b, C.byref(ms)))
         return ms.value
+
+    def last_steps_ms(self) -> float:
+        """Device time of the time-step loop of the last kbg_step / kbg_batch_step call."""
+        return self.elapsed_ms(_lib.KBG_EV_STEPS_BEGIN, _lib.KBG_EV_STEPS_END)
 
     def close(self):
         if self.handle:
