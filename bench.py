@@ -11,7 +11,7 @@ Printed (one JSON line on stdout):
   fp64_plain     the same instances stepped one solve at a time in FP64 (kbg_step per solve)
   l2_vs_fp64     max over instances of the relative L2 error in rho, U, T of the FP32 batch
                  against the FP64 single-solve results after the same steps
-  roofline       HBM roofline of the march kernels: algorithmic bytes per step (8 B per
+  roofline       HBM roofline of the sweep kernels: algorithmic bytes per step (8 B per
                  cell x velocity + 48 B per cell) / device time per step (HIP events on the
                  library's stream around the step loop)
   cpu_baseline   the FP64 CPU oracle (OpenMP) on a bounded sample of the same instances
@@ -63,10 +63,10 @@ def build_instances(kb, n):
 
 
 def traffic_from_profiles(points):
-    """HBM bytes per step of the march kernels from the committed rocprofv3 PMC summary
+    """HBM bytes per step of the sweep kernels from the committed rocprofv3 PMC summary
     (FETCH_SIZE + WRITE_SIZE per step, scaled by the copy-kernel calibration), when it was taken on
     this same workload (same number of cell x velocity points)."""
-    p = PROFILES / "march1d_fp32_batch512_pmc.txt"
+    p = PROFILES / "sweep1d_fp32_batch512_pmc.txt"
     if not p.exists():
         return None
     vals = {}
@@ -231,7 +231,7 @@ def main():
         "p99_solve_latency_ms": None,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_march1d<float,1F,KFVS,VPT 1..4>: one launch per lane mapping per step",
+                     "kernel": "k_sweep1d<float,1F,KFVS>: one launch per wavefront count (1..8) per step",
                      "algorithmic_bytes_per_step": alg_bytes_step, "device_ms_per_step": ms_step},
         "cpu_baseline": cpu,
     }

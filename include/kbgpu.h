@@ -84,9 +84,10 @@ const char *kbg_last_error(void);
 int kbg_ctx_create(int device, kbg_ctx **out);
 int kbg_ctx_destroy(kbg_ctx *ctx);
 int kbg_ctx_sync(kbg_ctx *ctx);
-/* Fused-kernel schedule: cells per work item and wavefronts launched (0 = one per item). */
-int kbg_ctx_set_schedule(kbg_ctx *ctx, int seg_cells, int waves);
-int kbg_ctx_get_schedule(kbg_ctx *ctx, int *seg_cells, int *waves);
+/* Schedule of the fused 1D sweep: cells per workgroup (1..60, default 60) and the number of
+ * concurrent streams a batch spreads its kernel groups over (1..4; 0 = all 4). */
+int kbg_ctx_set_schedule(kbg_ctx *ctx, int seg_cells, int streams);
+int kbg_ctx_get_schedule(kbg_ctx *ctx, int *seg_cells, int *streams);
 /* HIP events on the context stream, for timing: record slot (0..15), elapsed ms between slots.
  * kbg_step / kbg_batch_step record slots KBG_EV_STEPS_BEGIN and KBG_EV_STEPS_END around their
  * time-step loop (after the descriptor upload, before the residual download), so the elapsed

@@ -1,7 +1,7 @@
 // kbg_1d.hip -- 1D discrete-velocity kernels for gfx950 (phase-exact path).
 //
 // Two paths share the physics (kbg_device.h):
-//  * k_march1d (kbg_march.hip): the fused time step.
+//  * k_sweep1d (kbg_sweep.h): the fused time step of one-velocity-dimension solves.
 //  * k_recon1d / k_evolve1d / k_wall1d / k_update1d: one workgroup per cell or face, the three
 //    phases kept separate exactly as KitBase runs them, with slopes and face fluxes stored in HBM.
 //    These back the phase-level entry points and handle any velocity count (1F3V included).
@@ -9,6 +9,18 @@
 #include "kbg_types.h"
 
 namespace kbg {
+
+// One cell's distributions in the solve's f layout: element o = q nv + v (q = 0 h, 1 b)
+template <class T> struct FRow {
+    T* p;
+    int nv;
+    long qs, vs;
+    KBG_D T& operator[](int o) const {
+        const int q = o >= nv ? 1 : 0;
+        return p[q * qs + (long)(o - q * nv) * vs];
+    }
+};
+template <class T> KBG_D FRow<T> frow(const Solve1D& S, T* f, int k) { return FRow<T>{f + (long)k * S.fcs, S.nv, S.fqs, S.fvs}; }
 
 // ------------------------------------------------------------------------------------------------
 // phase-exact kernels (one workgroup of 256 threads per cell or face)
@@ -27,10 +39,11 @@ template <class T> __global__ __launch_bounds__(256) void k_recon1d(const Solve1
                                                    (w[(size_t)(k + 1) * nw + m] - w[(size_t)k * nw + m]) / dR);
     }
     const T* f = (const T*)S.f[S.cur];
+    const FRow<const T> fa = frow(S, f, k - 1), fc = frow(S, f, k), fe = frow(S, f, k + 1);
     T* sf = (T*)S.sf;
     const T rL = T(1.0 / dL), rR = T(1.0 / dR);
     for (int q = threadIdx.x; q < RS; q += blockDim.x) {
-        const T a = f[(size_t)(k - 1) * RS + q], c = f[(size_t)k * RS + q], e = f[(size_t)(k + 1) * RS + q];
+        const T a = fa[q], c = fc[q], e = fe[q];
         sf[(size_t)k * RS + q] = limf<T>(S.limiter, (c - a) * rL, (e - c) * rR);
     }
 }
@@ -57,8 +70,7 @@ template <class T, int NF, int VD, int FLUX> __global__ __launch_bounds__(256) v
     double* fw = S.fw + (size_t)(k - 1) * NW;
     const double dt = S.dt, K = S.K;
     const T dtT = T(dt), hdt2 = T(0.5 * dt * dt);
-    const T* fL = f + (size_t)kL * RS;
-    const T* fR = f + (size_t)kR * RS;
+    const FRow<const T> fL = frow(S, f, kL), fR = frow(S, f, kR);
     const T* sL = sf + (size_t)kL * RS;
     const T* sR = sf + (size_t)kR * RS;
     double wL[NW], wR[NW];
@@ -216,8 +228,7 @@ template <class T> __global__ __launch_bounds__(256) void k_wall1d(const Solve1D
     const int rot = side == 0 ? 1 : -1;
     const int k = cell - 1 + S.ng;
     const int buf = fused ? ((S.cur + par) & 1) : S.cur;
-    const T* h = (const T*)S.f[buf] + (size_t)k * RS;
-    const T* b = h + nv;
+    const FRow<const T> h = frow(S, (const T*)S.f[buf], k);
     const T* U = (const T*)S.u;
     const T* WT = (const T*)S.wt;
     const double* bc = S.bcprim + side * 3;
@@ -245,7 +256,7 @@ template <class T> __global__ __launch_bounds__(256) void k_wall1d(const Solve1D
         const T c = x - Uw;
         const T H = A * texp(-lw * (c * c));
         const T hW = d ? H : h[iv];
-        const T bW = d ? H * kb : b[iv];
+        const T bW = d ? H * kb : h[nv + iv];
         m0.add(wq * x * hW);
         m1.add(wq * (x * x) * hW);
         m2.add(wq * x * (hW * (x * x)));
@@ -274,7 +285,7 @@ template <class T, int NF, int VD> __global__ __launch_bounds__(256) void k_upda
     const int k = c - 1 + ng;
     double* w = S.w[S.cur] + (size_t)k * NW;
     double* prim = S.prim + (size_t)k * NW;
-    T* f = (T*)S.f[S.cur] + (size_t)k * RS;
+    const FRow<T> f = frow(S, (T*)S.f[S.cur], k);
     const T* ffL = (const T*)S.ff + (size_t)(c - 1) * RS;
     const T* ffR = (const T*)S.ff + (size_t)c * RS;
     const double* fwL = S.fw + (size_t)(c - 1) * NW;

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../../include/kbgpu.h"
+#include "kbg_sweep.h"
 #include "kbg_types.h"
 
 using kbg::Solve1D;
@@ -34,8 +35,7 @@ int fail(int code, const std::string& msg) {
         if (e_ != hipSuccess) return fail(KBG_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-constexpr int kMaxVpt = 4;    // fused 1D kernel: up to 4 velocities per lane (nv <= 256)
-constexpr int kMaxSeg = 252;  // fused 1D kernel: cells per work item (LDS slice bound)
+constexpr int kMaxSeg = kbg::kSweepOut;  // fused 1D sweep: cells per work item
 
 }  // namespace
 
@@ -48,8 +48,8 @@ struct kbg_ctx {
     static constexpr int kSide = 3;
     hipStream_t side[kSide] = {};
     hipEvent_t fork = nullptr, join[kSide] = {};
-    int seg_cells = 64;
-    int waves = 0;
+    int seg_cells = kbg::kSweepOut;  // cells per sweep workgroup
+    int streams = 0;                 // concurrent streams of a batch (0: all, 1 + kSide)
     // batch staging (device)
     Solve1D* d_descs = nullptr;
     size_t descs_cap = 0;
@@ -66,7 +66,10 @@ struct kbg_solve {
     bool fp64 = false;
     size_t esz = 4;
     int nw = 3, N = 0, RS = 0;
-    bool fused = false;
+    bool fused = false;   // stepped by the fused sweep (cell-fastest f)
+    int sweep_nw = 0;     // wavefronts per sweep workgroup
+    double vtab_dt = -1;  // dt the velocity table was built for
+    void* vtab = nullptr;
     // common device buffers
     void *u = nullptr, *v = nullptr, *wv = nullptr, *wt = nullptr;
     double *dx = nullptr, *dy = nullptr;
@@ -75,10 +78,6 @@ struct kbg_solve {
     // 1D
     Solve1D h1{};
     Solve1D* d1 = nullptr;
-    std::vector<WorkItem> items;
-    int items_seg = 0;
-    WorkItem* d_items = nullptr;
-    double* d_rpart = nullptr;
     // 2D
     Solve2D h2{};
     Solve2D* d2 = nullptr;
@@ -131,6 +130,87 @@ int put_f64(kbg_solve* s, double* dst, const double* src, size_t n) {
 int get_f64(kbg_solve* s, double* dst, const double* src, size_t n) {
     HIPCHK(hipMemcpyAsync(dst, src, n * 8, hipMemcpyDeviceToHost, s->ctx->stream));
     HIPCHK(hipStreamSynchronize(s->ctx->stream));
+    return KBG_OK;
+}
+
+// host f [N][nf][nv] (FP64) <-> a 1D solve's device f in its own layout and precision
+template <class T> void to_layout(const Solve1D& S, int N, const double* src, T* dst) {
+    const int nf = S.nf, nv = S.nv;
+    for (int k = 0; k < N; k++)
+        for (int q = 0; q < nf; q++)
+            for (int v = 0; v < nv; v++)
+                dst[(size_t)k * S.fcs + (size_t)q * S.fqs + (size_t)v * S.fvs] = T(src[((size_t)k * nf + q) * nv + v]);
+}
+template <class T> void from_layout(const Solve1D& S, int N, const T* src, double* dst) {
+    const int nf = S.nf, nv = S.nv;
+    for (int k = 0; k < N; k++)
+        for (int q = 0; q < nf; q++)
+            for (int v = 0; v < nv; v++)
+                dst[((size_t)k * nf + q) * nv + v] = double(src[(size_t)k * S.fcs + (size_t)q * S.fqs + (size_t)v * S.fvs]);
+}
+size_t f_elems_1d(const Solve1D& S, int N) { return S.swept ? (size_t)S.nf * S.nv * S.np : (size_t)N * S.nf * S.nv; }
+
+int put_f1d(kbg_solve* s, void* dst, const double* src) {
+    const Solve1D& S = s->h1;
+    const size_t n = f_elems_1d(S, s->N);
+    if (s->fp64) {
+        std::vector<double> tmp(n, 0.0);
+        to_layout<double>(S, s->N, src, tmp.data());
+        HIPCHK(hipMemcpyAsync(dst, tmp.data(), n * 8, hipMemcpyHostToDevice, s->ctx->stream));
+        HIPCHK(hipStreamSynchronize(s->ctx->stream));
+    } else {
+        std::vector<float> tmp(n, 0.0f);
+        to_layout<float>(S, s->N, src, tmp.data());
+        HIPCHK(hipMemcpyAsync(dst, tmp.data(), n * 4, hipMemcpyHostToDevice, s->ctx->stream));
+        HIPCHK(hipStreamSynchronize(s->ctx->stream));
+    }
+    return KBG_OK;
+}
+
+int get_f1d(kbg_solve* s, double* dst, const void* src) {
+    const Solve1D& S = s->h1;
+    const size_t n = f_elems_1d(S, s->N);
+    if (s->fp64) {
+        std::vector<double> tmp(n);
+        HIPCHK(hipMemcpyAsync(tmp.data(), src, n * 8, hipMemcpyDeviceToHost, s->ctx->stream));
+        HIPCHK(hipStreamSynchronize(s->ctx->stream));
+        from_layout<double>(S, s->N, tmp.data(), dst);
+    } else {
+        std::vector<float> tmp(n);
+        HIPCHK(hipMemcpyAsync(tmp.data(), src, n * 4, hipMemcpyDeviceToHost, s->ctx->stream));
+        HIPCHK(hipStreamSynchronize(s->ctx->stream));
+        from_layout<float>(S, s->N, tmp.data(), dst);
+    }
+    return KBG_OK;
+}
+
+// per-velocity constants of the sweep for time step dt (kbg_sweep.h VEnt)
+template <class T> void build_vtab(const kbg_desc& d, double dt, std::vector<char>& out) {
+    const int n = d.nv + 8;
+    out.assign(sizeof(kbg::VEnt<T>) * n, 0);
+    auto* e = reinterpret_cast<kbg::VEnt<T>*>(out.data());
+    for (int v = 0; v < n; v++) {
+        e[v].up = 1;
+        if (v >= d.nv) continue;
+        const double u = d.u[v], w = d.weights[v];
+        double p = w;
+        for (int m = 0; m < 5; m++, p *= u) e[v].wk[m] = T(p);
+        e[v].u = T(u);
+        e[v].uu = T(u * u);
+        e[v].cu = T(dt * u);
+        e[v].cs = T(-0.5 * dt * dt * (u * u));
+        e[v].up = u >= 0.0 ? 1 : 0;
+    }
+}
+
+int ensure_vtab(kbg_solve* s, double dt) {
+    if (!s->fused || s->vtab_dt == dt) return KBG_OK;
+    std::vector<char> tab;
+    if (s->fp64) build_vtab<double>(s->d, dt, tab);
+    else build_vtab<float>(s->d, dt, tab);
+    HIPCHK(hipMemcpyAsync(s->vtab, tab.data(), tab.size(), hipMemcpyHostToDevice, s->ctx->stream));
+    HIPCHK(hipStreamSynchronize(s->ctx->stream));
+    s->vtab_dt = dt;
     return KBG_OK;
 }
 
@@ -214,6 +294,20 @@ void fill1d(kbg_solve* s) {
     S.om = d.omega;
     S.Pr = d.Pr;
     S.dt = 0.0;
+    // the fused sweep takes one velocity dimension and up to 16 wavefronts of velocities
+    const int N = d.nx + 2 * d.ng;
+    S.swept = d.vdim == 1 && d.nv <= kbg::sweep_max_nv(d.precision == KBG_FP64, d.nf);
+    if (S.swept) {
+        S.np = (N + 31) / 32 * 32;
+        S.fcs = 1;
+        S.fvs = S.np;
+        S.fqs = (long)d.nv * S.np;
+    } else {
+        S.np = N;
+        S.fcs = (long)d.nf * d.nv;
+        S.fqs = d.nv;
+        S.fvs = 1;
+    }
 }
 
 void fill2d(kbg_solve* s) {
@@ -291,22 +385,17 @@ int sync_boundary_faces(kbg_solve* s) {
     return KBG_OK;
 }
 
+// segments of the stepped cells, in order along x (neighbouring segments share halo rows)
 void build_items(kbg_solve* s, int seg, int solve_index, std::vector<WorkItem>& out) {
     const Solve1D& S = s->h1;
-    const int vpt = (S.nv + 63) / 64;
     for (int c0 = S.upd_lo; c0 <= S.upd_hi; c0 += seg) {
         WorkItem wi;
         wi.solve = solve_index;
         wi.c0 = c0;
         wi.c1 = std::min(c0 + seg - 1, S.upd_hi);
-        wi.vpt = vpt;
+        wi.nw = s->sweep_nw;
         out.push_back(wi);
     }
-}
-
-int grid_for(kbg_ctx* c, int nitems) {
-    int waves = c->waves > 0 ? std::min(c->waves, nitems) : nitems;
-    return std::max(1, (waves + 3) / 4);
 }
 
 int step_phase_1d(kbg_solve* s, double dt, double* residual);
@@ -373,21 +462,22 @@ int kbg_ctx_sync(kbg_ctx* c) {
     return KBG_OK;
 }
 
-int kbg_ctx_set_schedule(kbg_ctx* c, int seg_cells, int waves) {
+int kbg_ctx_set_schedule(kbg_ctx* c, int seg_cells, int streams) {
     if (!c) return fail(KBG_ERR_ARG, "ctx is NULL");
-    // the fused kernel stages (seg_cells + 4) x 8 doubles per wavefront in LDS: 4 wavefronts per
-    // workgroup keep that under 64 KiB
-    if (seg_cells < 1 || seg_cells > kMaxSeg || waves < 0)
-        return fail(KBG_ERR_ARG, "schedule: seg_cells must be in [1, " + std::to_string(kMaxSeg) + "], waves >= 0");
+    // a sweep workgroup's wavefronts hold 64 cells: the segment plus two halo cells on each side
+    if (seg_cells < 1 || seg_cells > kMaxSeg)
+        return fail(KBG_ERR_ARG, "schedule: seg_cells must be in [1, " + std::to_string(kMaxSeg) + "]");
+    if (streams < 0 || streams > 1 + kbg_ctx::kSide)
+        return fail(KBG_ERR_ARG, "schedule: streams must be in [0, " + std::to_string(1 + kbg_ctx::kSide) + "]");
     c->seg_cells = seg_cells;
-    c->waves = waves;
+    c->streams = streams;
     return KBG_OK;
 }
 
-int kbg_ctx_get_schedule(kbg_ctx* c, int* seg_cells, int* waves) {
+int kbg_ctx_get_schedule(kbg_ctx* c, int* seg_cells, int* streams) {
     if (!c) return fail(KBG_ERR_ARG, "ctx is NULL");
     if (seg_cells) *seg_cells = c->seg_cells;
-    if (waves) *waves = c->waves;
+    if (streams) *streams = c->streams;
     return KBG_OK;
 }
 
@@ -410,7 +500,11 @@ size_t kbg_solve_bytes(const kbg_desc* d) {
     const int nw = nw_of(d);
     if (d->dim == 1) {
         const size_t N = size_t(d->nx) + 2 * size_t(d->ng);
-        return 2 * N * d->nf * d->nv * esz + 3 * N * nw * 8 + N * 8 + 4 * size_t(d->nv) * esz + 2 * d->nf * d->nv * esz;
+        const bool swept = d->vdim == 1 && d->nv <= kbg::sweep_max_nv(d->precision == KBG_FP64, d->nf);
+        const size_t np = swept ? (N + 31) / 32 * 32 : N;
+        const size_t vt = swept ? (size_t(d->nv) + 8) * (esz == 8 ? sizeof(kbg::VEnt<double>) : sizeof(kbg::VEnt<float>)) : 0;
+        return 2 * np * d->nf * d->nv * esz + 3 * N * nw * 8 + N * 8 + 4 * size_t(d->nv) * esz + 2 * d->nf * d->nv * esz +
+               N * 4 * nw * 8 + vt;
     }
     const size_t C = size_t(d->nx) * d->ny, F = size_t(d->nx + 1) * d->ny + size_t(d->nx) * (d->ny + 1);
     return C * 2 * d->nv * esz * 3 + F * 2 * d->nv * esz + C * (2 * 4 * 8 + 8 * 8 + 16 * 8 + 16) + F * (4 + 3 + 4) * 8 +
@@ -452,13 +546,14 @@ int kbg_solve_create(kbg_ctx* ctx, const kbg_desc* desc, kbg_solve** out) {
         if ((rc = dev_alloc(s, &p, N * 8))) return bail(rc);
         s->dx = (double*)p;
         if ((rc = put_f64(s, s->dx, d.dx, N))) return bail(rc);
+        const size_t fel = f_elems_1d(S, (int)N);
         for (int b = 0; b < 2; b++) {
             if ((rc = dev_alloc(s, &p, N * s->nw * 8))) return bail(rc);
             S.w[b] = (double*)p;
             HIPCHK(hipMemset(S.w[b], 0, N * s->nw * 8));
-            if ((rc = dev_alloc(s, &p, N * s->RS * s->esz))) return bail(rc);
+            if ((rc = dev_alloc(s, &p, fel * s->esz))) return bail(rc);
             S.f[b] = p;
-            HIPCHK(hipMemset(S.f[b], 0, N * s->RS * s->esz));
+            HIPCHK(hipMemset(S.f[b], 0, fel * s->esz));
         }
         if ((rc = dev_alloc(s, &p, N * s->nw * 8))) return bail(rc);
         S.prim = (double*)p;
@@ -487,7 +582,14 @@ int kbg_solve_create(kbg_ctx* ctx, const kbg_desc* desc, kbg_solve** out) {
         s->res = (double*)p;
         if ((rc = dev_alloc(s, &p, sizeof(Solve1D)))) return bail(rc);
         s->d1 = (Solve1D*)p;
-        s->fused = d.vdim == 1 && d.nv <= 64 * kMaxVpt;
+        s->fused = S.swept != 0;
+        if (s->fused) {
+            s->sweep_nw = (d.nv + kbg::sweep_vw(s->fp64, d.nf) - 1) / kbg::sweep_vw(s->fp64, d.nf);
+            const size_t vb = (size_t(d.nv) + 8) * (s->fp64 ? sizeof(kbg::VEnt<double>) : sizeof(kbg::VEnt<float>));
+            if ((rc = dev_alloc(s, &s->vtab, vb))) return bail(rc);
+            HIPCHK(hipMemset(s->vtab, 0, vb));
+            S.vtab = s->vtab;
+        }
         if ((rc = push1d(s))) return bail(rc);
     } else {
         const size_t C = size_t(d.nx) * d.ny, FX = size_t(d.nx + 1) * d.ny, FY = size_t(d.nx) * (d.ny + 1);
@@ -565,7 +667,7 @@ int kbg_upload(kbg_solve* s, const double* w, const double* prim, const double* 
         Solve1D& S = s->h1;
         for (int b = 0; b < 2; b++) {
             if ((rc = put_f64(s, S.w[b], w, N * s->nw))) return rc;
-            if ((rc = put_vec(s, S.f[b], f, N * s->RS))) return rc;
+            if ((rc = put_f1d(s, S.f[b], f))) return rc;
         }
         if ((rc = put_f64(s, S.prim, prim, N * s->nw))) return rc;
         return KBG_OK;
@@ -584,7 +686,7 @@ int kbg_download(kbg_solve* s, double* w, double* prim, double* f) {
         Solve1D& S = s->h1;
         if (w && (rc = get_f64(s, w, S.w[S.cur], N * s->nw))) return rc;
         if (prim && (rc = get_f64(s, prim, S.prim, N * s->nw))) return rc;
-        if (f && (rc = get_vec(s, f, S.f[S.cur], N * s->RS))) return rc;
+        if (f && (rc = get_f1d(s, f, S.f[S.cur]))) return rc;
         return KBG_OK;
     }
     Solve2D& S = s->h2;
@@ -739,24 +841,30 @@ int kbg_batch_step(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double* 
     for (int i = 0; i < n; i++) {
         kbg_solve* s = solves[i];
         if (!s || s->ctx != ctx) return fail(KBG_ERR_ARG, "solve is NULL or belongs to another context");
-        if (s->d.dim == 1 && s->fused) groups[{s->fp64 ? 1 : 0, s->d.nf, s->d.flux, (s->d.nv + 63) / 64}].push_back(i);
-        else others.push_back(i);
+        if (s->d.dim == 1 && s->fused) {
+            groups[{s->fp64 ? 1 : 0, s->d.nf, s->d.flux, s->sweep_nw}].push_back(i);
+            if ((rc = ensure_vtab(s, dts[i]))) return rc;
+        } else {
+            others.push_back(i);
+        }
     }
     // descriptors and work items of all fused solves
     std::vector<Solve1D> descs;
     std::vector<int> desc_of(n, -1);
     struct Group {
         bool fp64;
-        int nf, flux, vpt;
+        int nf, flux, nw;
         size_t item0, nitems;
         std::vector<int> members;  // solve indices
+        double work;               // wavefront-segments per step
+        int stream;                // 0: the context stream, k > 0: side stream k - 1
     };
     std::vector<Group> gl;
     std::vector<WorkItem> items;
     const int seg = std::max(1, ctx->seg_cells);
     for (auto& kv : groups) {
         Group g{std::get<0>(kv.first) == 1, std::get<1>(kv.first), std::get<2>(kv.first), std::get<3>(kv.first), items.size(), 0,
-                kv.second};
+                kv.second, 0.0, 0};
         std::vector<WorkItem> gi;
         for (int i : kv.second) {
             kbg_solve* s = solves[i];
@@ -765,13 +873,23 @@ int kbg_batch_step(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double* 
             build_items(s, seg, desc_of[i], gi);
             descs.push_back(s->h1);
         }
-        // longest work first: the dispatcher hands out workgroups in order
-        std::stable_sort(gi.begin(), gi.end(), [](const WorkItem& a, const WorkItem& b) {
-            return (long)(a.c1 - a.c0 + 1) * a.vpt > (long)(b.c1 - b.c0 + 1) * b.vpt;
-        });
         g.nitems = gi.size();
+        g.work = double(g.nitems) * g.nw;
         items.insert(items.end(), gi.begin(), gi.end());
         gl.push_back(g);
+    }
+    // heaviest groups first, each to the least-loaded stream
+    std::sort(gl.begin(), gl.end(), [](const Group& a, const Group& b) { return a.work > b.work; });
+    {
+        double load[1 + kbg_ctx::kSide] = {};
+        const int ns = std::min<int>(ctx->streams > 0 ? ctx->streams : 1 + kbg_ctx::kSide, (int)gl.size());
+        for (Group& g : gl) {
+            int best = 0;
+            for (int k = 1; k < ns; k++)
+                if (load[k] < load[best]) best = k;
+            g.stream = best;
+            load[best] += g.work;
+        }
     }
     if (!descs.empty()) {
         if (descs.size() > ctx->descs_cap) {
@@ -793,12 +911,13 @@ int kbg_batch_step(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double* 
     HIPCHK(hipEventRecord(ctx->ev[KBG_EV_STEPS_BEGIN], st));
     // group gi runs on stream gst[gi]; groups share no data, so they only fork from and join
     // back into the context stream
-    const int nside = std::min<int>(kbg_ctx::kSide, (int)gl.size() - 1);
+    int nside = 0;
+    for (const Group& g : gl) nside = std::max(nside, g.stream);
     std::vector<hipStream_t> gst(gl.size(), st);
     if (nside > 0) {
         HIPCHK(hipEventRecord(ctx->fork, st));
         for (int k = 0; k < nside; k++) HIPCHK(hipStreamWaitEvent(ctx->side[k], ctx->fork, 0));
-        for (size_t gi = 1; gi < gl.size(); gi++) gst[gi] = ctx->side[(gi - 1) % kbg_ctx::kSide];
+        for (size_t gi = 0; gi < gl.size(); gi++) gst[gi] = gl[gi].stream == 0 ? st : ctx->side[gl[gi].stream - 1];
     }
     for (int t = 0; t < nsteps; t++) {
         const int par = t & 1;
@@ -811,8 +930,8 @@ int kbg_batch_step(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double* 
                     HIPCHK(kbg::launch_wall1d(solves[di]->fp64, S, ctx->d_descs + desc_of[di], 1, par, gst[gi]));
             }
             double* rp = (last && residuals) ? ctx->d_rpart + g.item0 * 12 : nullptr;
-            HIPCHK(kbg::launch_march1d(g.fp64, g.nf, g.flux, g.vpt, ctx->d_descs, ctx->d_items + g.item0, (int)g.nitems,
-                                       grid_for(ctx, (int)g.nitems), par, rp, seg, gst[gi]));
+            HIPCHK(kbg::launch_sweep1d(g.fp64, g.nf, g.flux, g.nw, ctx->d_descs, ctx->d_items + g.item0, (int)g.nitems, par,
+                                       rp, gst[gi]));
         }
         for (int i : others) {
             kbg_solve* s = solves[i];

@@ -1,0 +1,512 @@
+// kbg_sweep.h -- the fused 1D time step (reconstruct! + evolve! + update! in one pass) for gfx950.
+//
+// Layout.  A swept solve stores its distributions cell-fastest: f[q][v][k] at ((q nv + v) np + k)
+// for distribution q (h, b), velocity v and storage cell k, each velocity row padded to np cells
+// (a multiple of 32, so every row starts on a 128-byte line).
+//
+// Work decomposition.  One workgroup per segment of up to 60 consecutive cells of one solve.  Its NW
+// wavefronts all hold the same 64 storage cells -- lane L is cell k0 + L: the segment plus two halo
+// cells on each side -- and split the velocity grid: wavefront w takes velocities
+// [w VW, (w + 1) VW).  Neighbouring cells are neighbouring lanes, reached with DPP whole-wavefront
+// shifts, so the stencil needs no LDS and no extra loads.
+//
+//   pass 1, per velocity: one coalesced load of the 64 cells' f, the limited slope (reconstruct!),
+//     the upwind flux of the lane's left face (evolve!), its moment sums in FP64, and the transport
+//     update G = f + (ff_left - ff_right) / dx, kept in a register;
+//   the wavefronts' moment sums are added through LDS in a fixed order;
+//   every lane runs its own cell's FP64 chain (step!): w update, conserve_prim, tau, equilibrium and
+//     Shakhov coefficients -- 64 cells per pass;
+//   pass 2, per velocity: relax G toward the new equilibrium and store f^{n+1}.
+//
+// f^n is read once and f^{n+1} written once (plus the 4-cell halo per segment): the compulsory HBM
+// traffic of a time step.  The per-cell FP64 work is lane-parallel over the cells.
+//
+// Reference semantics (KitBase.jl v0.9.31):
+//   slopes    solver_reconstruction.jl:56-153 (reconstruct3! reconstruction.jl:136-151, limiters
+//             recon_limiter.jl:10-105)
+//   faces     solver_evolution.jl:39-89; flux_kfvs.jl:238-340; flux_kcu.jl:241-303,463-530;
+//             UGKS flux_ugks.jl:8-114 with the reconstructed wL = w + dx/2 sw (solver_evolution.jl:285-312)
+//   walls     evolve_boundary.jl:4-47 (boundary-face buffers written by k_wall1d)
+//   cells     solver_update.jl:61-93, update_boundary.jl:6-61; step_1f.jl:6-55, step_2f.jl:6-64
+#pragma once
+
+#include "kbg_device.h"
+#include "kbg_types.h"
+
+namespace kbg {
+
+constexpr int kSweepLanes = 64;  // storage cells held by a wavefront
+constexpr int kSweepOut = 60;    // cells a segment updates (lanes 2 .. 61)
+
+// DPP whole-wavefront shifts: shr1(x)[L] = x[L - 1], shl1(x)[L] = x[L + 1] (0 beyond the ends)
+KBG_D float shr1(float x) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x138, 0xf, 0xf, false));
+}
+KBG_D float shl1(float x) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x130, 0xf, 0xf, false));
+}
+KBG_D double shr1(double x) {
+    const long long b = __builtin_bit_cast(long long, x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, 0x138, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x138, 0xf, 0xf, false);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);
+}
+KBG_D double shl1(double x) {
+    const long long b = __builtin_bit_cast(long long, x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, 0x130, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x130, 0xf, 0xf, false);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// exp2 in the storage precision (the FP32 path uses the hardware exp2 directly)
+KBG_D float texp2(float x) { return __builtin_amdgcn_exp2f(x); }
+KBG_D double texp2(double x) { return exp2(x); }
+// reciprocal in the storage precision
+KBG_D float trcp(float x) { return __builtin_amdgcn_rcpf(x); }
+KBG_D double trcp(double x) { return 1.0 / x; }
+
+constexpr double kLog2e = 1.4426950408889634074;
+
+// Per-velocity constants of a swept solve, rebuilt on the host when dt changes.  Read through the
+// scalar cache (the velocity index is uniform across a wavefront).  Entries past nv are zero (with
+// up = 1), so padded velocities add nothing to the sums and are never stored.
+template <class T> struct VEnt {
+    T wk[5];   // quadrature weight x u^k, k = 0..4
+    T u, uu;   // u, u^2
+    T cu, cs;  // dt u, -dt^2 u^2 / 2 (KFVS)
+    int up;    // u >= 0: the flux takes the left state (heaviside, math_general.jl:13)
+    int pad_[2];
+};
+
+// velocities per wavefront and moment-sum counts of a kernel instance
+template <class T, int NF, int FLUX> struct SweepCfg {
+    static constexpr int VW = (sizeof(T) == 4 ? 32 : 16) / NF;
+    static constexpr int CH = VW < 8 ? VW : 8;  // velocities loaded together
+    static constexpr int NRAW = FLUX == UGKS ? 6 : (NF == 2 ? 4 : 3);
+    static constexpr int NR = NRAW + 1;  // + the Shakhov heat-flux sum
+};
+
+// limiter family of a kernel path: the van Leer default and first order are specialised, the others
+// take the generic runtime switch
+enum { LIM_VANLEER = 0, LIM_GENERIC = 1, LIM_NONE = 2 };
+
+template <class T, int LIMK> KBG_D T sweep_limit(int kind, T sL, T sR) {
+    if constexpr (LIMK == LIM_NONE) {
+        return T(0);
+    } else if constexpr (LIMK == LIM_VANLEER) {
+        // (sgn sL + sgn sR) |sL| |sR| / (|sL| + |sR| + 1e-7) (recon_limiter.jl:17-19): zero unless
+        // sL sR > 0, and then 2 sL sR / (sL + sR + copysign(1e-7, sL))
+        const T p = sL * sR;
+        const T den = sL + sR + copysign(T(1e-7), sL);
+        const T r = T(2) * p * trcp(den);
+        return p > T(0) ? r : T(0);
+    } else {
+        return limiter<T>(kind, sL, sR);
+    }
+}
+
+// face coefficients of the lane's left face, in the storage precision (KCU / UGKS)
+template <class T> struct SFace {
+    T A, nl2, Ui, kb, Mxi2, M0, M1, M2, M3, M4, aL0, aL1, aL2, aR0, aR1, aR2, aT0, aT1, aT2;
+};
+
+template <class T, int NF, int FLUX, int LIMK, bool SHAK>
+KBG_D void sweep1d(const Solve1D& S, const WorkItem& wi, int par, int lane, int wave, int NW, double* lds,
+                   double* rrow) {
+    using C = SweepCfg<T, NF, FLUX>;
+    constexpr int VW = C::VW, CH = C::CH, NRAW = C::NRAW, NR = C::NR;
+    const int nx = S.nx, ng = S.ng, nv = S.nv, N = nx + 2 * ng;
+    const long np = S.np;
+    const int in = (S.cur + par) & 1;
+    const gptr<const T> fin = to_global((const T*)S.f[in]);
+    const gptr<T> fout = to_global((T*)S.f[in ^ 1]);
+    // read-only for the kernel's lifetime: the constant address space gives scalar (SMEM) loads
+    const __attribute__((address_space(4))) VEnt<T>* vt = (const __attribute__((address_space(4))) VEnt<T>*)S.vtab;
+    const double dt = S.dt, K = S.K;
+    const int lk = S.limiter;
+
+    // ---- per-lane cell
+    const int k0 = wi.c0 - 1 + ng - 2;  // storage cell of lane 0
+    const int k = k0 + lane;
+    const int kc = k < 0 ? 0 : (k >= N ? N - 1 : k);  // clamped: lanes past the ends load valid memory
+    const int i = k - ng + 1;                          // physical index
+    const double d0 = S.dx[kc];
+    const double dm = S.dx[kc > 0 ? kc - 1 : 0], dp = S.dx[kc < N - 1 ? kc + 1 : N - 1];
+    const bool slope_on = S.interp2 && i >= S.slope_lo && i <= S.slope_hi && k >= 0 && k < N;
+    // zero slope distances give exactly zero slopes outside reconstruct!'s range
+    const double rL64 = slope_on ? 1.0 / (0.5 * (dm + d0)) : 0.0, rR64 = slope_on ? 1.0 / (0.5 * (d0 + dp)) : 0.0;
+    const T rL = T(rL64), rR = T(rR64);
+    const double dxh64 = 0.5 * d0, rdx64 = 1.0 / d0;
+    const T dxh = T(dxh64), rdx = T(rdx64);
+    const bool upd = lane >= 2 && lane < 2 + kSweepOut && i >= wi.c0 && i <= wi.c1;
+    // the lane's left face comes from the boundary-face buffers (only where a stepped cell uses it)
+    const bool bufL = (i == 1 && S.faceL_buf && S.upd_lo == 1) || (i == nx + 1 && S.faceR_buf && S.upd_hi == nx);
+    const int bside = i == 1 ? 0 : 1;
+    const bool seg_buf = __builtin_amdgcn_readfirstlane(__ballot(bufL) != 0ull ? 1 : 0) != 0;
+    double wo[3];
+    const gptr<const double> win = to_global((const double*)S.w[in]);
+#pragma unroll
+    for (int m = 0; m < 3; m++) wo[m] = win[(size_t)kc * 3 + m];
+    const bool shak = SHAK && i > 1 && i < nx;  // boundary cells relax with BGK (update_boundary.jl)
+    double po[3] = {1.0, 0.0, 1.0};
+    if (SHAK) {
+#pragma unroll
+        for (int m = 0; m < 3; m++) po[m] = S.prim[(size_t)kc * 3 + m];
+    }
+    const double Uo64 = po[1];
+
+    // ---- face coefficients of the lane's left face (KCU / UGKS), FP64 per lane
+    SFace<T> F{};
+    double fw0[3] = {0, 0, 0}, m3 = 0.0, m4 = 0.0;
+    if constexpr (FLUX != KFVS) {
+        double sw[3], wL[3], wR[3];
+        const double dxL = shr1(dxh64);
+#pragma unroll
+        for (int m = 0; m < 3; m++) {
+            const double wl = shr1(wo[m]), wr = shl1(wo[m]);
+            sw[m] = LIMK == LIM_NONE ? 0.0 : limiter<double>(lk, (wo[m] - wl) * rL64, (wr - wo[m]) * rR64);
+        }
+#pragma unroll
+        for (int m = 0; m < 3; m++) {
+            wL[m] = shr1(wo[m]) + shr1(sw[m]) * dxL;
+            wR[m] = wo[m] - sw[m] * dxh64;
+        }
+        if constexpr (FLUX == KCU) {
+            KcuFace<3> Fk;
+            kcu_face<3>(wL, wR, K, S.gam, S.mu, S.om, dt, Fk);
+            F.A = T(Fk.prim[0] * sqrt(Fk.prim[2] / kPi));
+            F.nl2 = T(-Fk.prim[2] * kLog2e);
+            F.Ui = T(Fk.prim[1]);
+            F.kb = T(K / (2.0 * Fk.prim[2]));
+            F.M0 = T(Fk.Mt1);
+            F.M1 = T(Fk.Mt2);
+            for (int m = 0; m < 3; m++) fw0[m] = Fk.fw0[m];
+            m3 = Fk.Mt2;
+        } else {
+            UgksFace1 Fu;
+            ugks_face_1d(wL, wR, K, S.gam, S.mu, S.om, dt, dxL, dxh64, Fu);
+            F.A = T(Fu.prim[0] * sqrt(Fu.prim[2] / kPi));
+            F.nl2 = T(-Fu.prim[2] * kLog2e);
+            F.Ui = T(Fu.prim[1]);
+            F.kb = T(K / (2.0 * Fu.prim[2]));
+            F.Mxi2 = T(Fu.Mxi2);
+            F.M0 = T(Fu.Mt[0]);
+            F.M1 = T(Fu.Mt[1]);
+            F.M2 = T(Fu.Mt[2]);
+            F.M3 = T(Fu.Mt[3]);
+            F.M4 = T(Fu.Mt[4]);
+            F.aL0 = T(Fu.aL[0]);
+            F.aL1 = T(Fu.aL[1]);
+            F.aL2 = T(Fu.aL[2]);
+            F.aR0 = T(Fu.aR[0]);
+            F.aR1 = T(Fu.aR[1]);
+            F.aR2 = T(Fu.aR[2]);
+            F.aT0 = T(Fu.aT[0]);
+            F.aT1 = T(Fu.aT[1]);
+            F.aT2 = T(Fu.aT[2]);
+            for (int m = 0; m < 3; m++) fw0[m] = Fu.fw0[m];
+            m3 = Fu.Mt[3];
+            m4 = Fu.Mt[4];
+        }
+    }
+
+    // ---- pass 1: slopes, face fluxes, moment sums, transport update
+    const int vbeg = wave * VW;
+    const int vend = min(nv, vbeg + VW);
+    T G[NF][VW];
+    double acc[NR];
+#pragma unroll
+    for (int m = 0; m < NR; m++) acc[m] = 0.0;
+    const gptr<const T> bff = to_global((const T*)S.bff);
+    const T UoT = T(Uo64);
+
+    // one chunk of CH velocities; UPM: 0 all take the right state, 1 all the left, 2 mixed;
+    // BUF: a lane's left face comes from the boundary-face buffers
+    auto chunk1 = [&](auto cc, auto upm, auto buf) {
+        constexpr int c = cc, UPM = upm;
+        constexpr bool BUF = buf;
+        const int vb = vbeg + c * CH;
+        T fr[NF][CH];
+        static_for<CH>([&](auto jc) {
+            constexpr int j = jc;
+            const int v = min(vb + j, nv - 1);
+#pragma unroll
+            for (int q = 0; q < NF; q++) fr[q][j] = fin[((long)q * nv + v) * np + kc];
+        });
+        T part[NR];  // chunk sums in the storage precision, added to the FP64 totals below
+#pragma unroll
+        for (int m = 0; m < NR; m++) part[m] = T(0);
+        static_for<CH>([&](auto jc) {
+            constexpr int j = jc;
+            const auto& e = vt[vb + j];
+            T fq[NF], sq[NF], fu[NF], su[NF], ffq[NF];
+#pragma unroll
+            for (int q = 0; q < NF; q++) {
+                const T f = fr[q][j];
+                fq[q] = f;
+                sq[q] = sweep_limit<T, LIMK>(lk, (f - shr1(f)) * rL, (shl1(f) - f) * rR);
+            }
+            // upwind state and slope of the left face (flux_kfvs.jl:238-264, :303-340)
+#pragma unroll
+            for (int q = 0; q < NF; q++) {
+                if constexpr (UPM == 1) {
+                    fu[q] = shr1(fq[q] + sq[q] * dxh);
+                    su[q] = shr1(sq[q]);
+                } else if constexpr (UPM == 0) {
+                    fu[q] = fq[q] - sq[q] * dxh;
+                    su[q] = sq[q];
+                } else {
+                    const T l = shr1(fq[q] + sq[q] * dxh), r = fq[q] - sq[q] * dxh;
+                    const T sl = shr1(sq[q]);
+                    fu[q] = e.up ? l : r;
+                    su[q] = e.up ? sl : sq[q];
+                }
+            }
+            if constexpr (FLUX == KFVS) {
+#pragma unroll
+                for (int q = 0; q < NF; q++) ffq[q] = fu[q] * e.cu + su[q] * e.cs;
+            } else if constexpr (FLUX == KCU) {
+                // flux_kcu.jl:241-303 (1F1V), :463-530 (2F1V)
+                const T cI = e.u - F.Ui;
+                const T g = F.A * texp2(F.nl2 * (cI * cI));
+                ffq[0] = F.M0 * e.u * g + F.M1 * e.u * fu[0];
+                if constexpr (NF == 2) ffq[1] = F.M0 * e.u * (g * F.kb) + F.M1 * e.u * fu[1];
+            } else {
+                // flux_ugks.jl:91-111 (1D2F1V)
+                const T x = e.u, uu = e.uu;
+                const T cI = x - F.Ui;
+                const T H = F.A * texp2(F.nl2 * (cI * cI));
+                const T Bq = H * F.kb;
+                T a0, a1, a2;
+                if constexpr (UPM == 1) {
+                    a0 = F.aL0, a1 = F.aL1, a2 = F.aL2;
+                } else if constexpr (UPM == 0) {
+                    a0 = F.aR0, a1 = F.aR1, a2 = F.aR2;
+                } else {
+                    a0 = e.up ? F.aL0 : F.aR0, a1 = e.up ? F.aL1 : F.aR1, a2 = e.up ? F.aL2 : F.aR2;
+                }
+                const T pH = a0 * H + a1 * x * H + T(0.5) * a2 * (uu * H + Bq);
+                const T pTH = F.aT0 * H + F.aT1 * x * H + T(0.5) * F.aT2 * (uu * H + Bq);
+                const T pB = a0 * Bq + a1 * x * Bq + T(0.5) * a2 * (uu * Bq + F.Mxi2 * H);
+                const T pTB = F.aT0 * Bq + F.aT1 * x * Bq + T(0.5) * F.aT2 * (uu * Bq + F.Mxi2 * H);
+                ffq[0] = F.M0 * x * H + F.M1 * uu * pH + F.M2 * x * pTH + F.M3 * x * fu[0] - F.M4 * uu * su[0];
+                ffq[NF - 1] = F.M0 * x * Bq + F.M1 * uu * pB + F.M2 * x * pTB + F.M3 * x * fu[NF - 1] - F.M4 * uu * su[NF - 1];
+            }
+            if constexpr (BUF) {
+                // a face with no neighbour to reconstruct from, or a wall: the boundary-face buffers
+                const int v = min(vb + j, nv - 1);
+#pragma unroll
+                for (int q = 0; q < NF; q++) {
+                    const T bv = bff[((long)bside * NF + q) * nv + v];
+                    ffq[q] = bufL ? bv : ffq[q];
+                }
+            }
+            // moment sums of the face
+            if constexpr (FLUX == KFVS) {
+                part[0] += e.wk[0] * ffq[0];
+                part[1] += e.wk[1] * ffq[0];
+                part[2] += e.wk[2] * ffq[0];
+                if constexpr (NF == 2) part[3] += e.wk[0] * ffq[1];
+            } else if constexpr (FLUX == KCU) {
+                part[0] += e.wk[1] * fu[0];
+                part[1] += e.wk[2] * fu[0];
+                part[2] += e.wk[3] * fu[0];
+                if constexpr (NF == 2) part[3] += e.wk[1] * fu[1];
+            } else {
+                part[0] += e.wk[1] * fu[0];
+                part[1] += e.wk[2] * su[0];
+                part[2] += e.wk[2] * fu[0];
+                part[3] += e.wk[3] * su[0];
+                part[4] += e.wk[3] * fu[0] + e.wk[1] * fu[1];
+                part[5] += e.wk[4] * su[0] + e.wk[2] * su[1];
+            }
+            if constexpr (SHAK) {
+                // heat flux of the cell from the old f and prim (theory_moments.jl:523, :530-531)
+                const T co = e.u - UoT;
+                const T wc = e.wk[0] * co;
+                part[NRAW] += (wc * (co * co)) * fq[0];
+                if constexpr (NF == 2) part[NRAW] += wc * fq[1];
+            }
+            // transport update G = f + (ff_left - ff_right) / dx
+#pragma unroll
+            for (int q = 0; q < NF; q++) G[q][c * CH + j] = fq[q] + (ffq[q] - shl1(ffq[q])) * rdx;
+        });
+#pragma unroll
+        for (int m = 0; m < NR; m++) acc[m] += double(part[m]);
+    };
+    using I0 = std::integral_constant<int, 0>;
+    using I1 = std::integral_constant<int, 1>;
+    using I2 = std::integral_constant<int, 2>;
+    using BT = std::integral_constant<bool, true>;
+    using BF = std::integral_constant<bool, false>;
+    static_for<VW / CH>([&](auto cc) {
+        constexpr int c = cc;
+        const int vb = vbeg + c * CH;
+        if (vb >= vend) return;
+        if (LIMK == LIM_VANLEER && !seg_buf) {
+            // velocities are sorted: a chunk whose ends agree in sign takes one side throughout
+            const int u0 = vt[vb].up, u1 = vt[min(vb + CH - 1, nv - 1)].up;
+            if (u0 && u1) chunk1(cc, I1{}, BF{});
+            else if (!u0 && !u1) chunk1(cc, I0{}, BF{});
+            else chunk1(cc, I2{}, BF{});
+        } else if (seg_buf) {
+            chunk1(cc, I2{}, BT{});
+        } else {
+            chunk1(cc, I2{}, BF{});
+        }
+    });
+
+    // ---- add the wavefronts' sums (fixed order)
+    double tot[NR];
+    if (NW > 1) {
+#pragma unroll
+        for (int m = 0; m < NR; m++) lds[(wave * NR + m) * kSweepLanes + lane] = acc[m];
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < NR; m++) {
+            double s = 0.0;
+            for (int w = 0; w < NW; w++) s += lds[(w * NR + m) * kSweepLanes + lane];
+            tot[m] = s;
+        }
+    } else {
+#pragma unroll
+        for (int m = 0; m < NR; m++) tot[m] = acc[m];
+    }
+
+    // ---- face moments of the lane's left face, and of its right face (the next lane's left)
+    double fwl[3];
+    if constexpr (FLUX == KFVS) {
+        fwl[0] = tot[0];
+        fwl[1] = tot[1];
+        fwl[2] = NF == 2 ? 0.5 * (tot[2] + tot[3]) : 0.5 * tot[2];
+    } else if constexpr (FLUX == KCU) {
+        fwl[0] = fw0[0] + m3 * tot[0];
+        fwl[1] = fw0[1] + m3 * tot[1];
+        fwl[2] = fw0[2] + m3 * 0.5 * (NF == 2 ? tot[2] + tot[3] : tot[2]);
+    } else {
+        fwl[0] = fw0[0] + (m3 * tot[0] - m4 * tot[1]);
+        fwl[1] = fw0[1] + (m3 * tot[2] - m4 * tot[3]);
+        fwl[2] = fw0[2] + (m3 * 0.5 * tot[4] - m4 * 0.5 * tot[5]);
+    }
+    if (bufL) {
+#pragma unroll
+        for (int m = 0; m < 3; m++) fwl[m] = S.bfw[bside * 3 + m];
+    }
+    double wn[3], pn[3];
+#pragma unroll
+    for (int m = 0; m < 3; m++) wn[m] = wo[m] + (fwl[m] - shl1(fwl[m])) * rdx64;
+
+    // ---- the cell's step! coefficients (FP64 chain, lane-parallel)
+    conserve_prim<3>(wn, S.gam, pn);
+    const T rT = relax_ratio<T>(pn[0], pn[2], S.mu, S.om, dt);
+    const T inv = T(1) / (T(1) + rT), rTi = rT * inv;
+    const T An = T(pn[0] * sqrt(pn[2] / kPi)), nl2 = T(-pn[2] * kLog2e), Un = T(pn[1]);
+    const T kbn = T(K / (2.0 * pn[2]));
+    T pre = 0, Ao = 0, nlo2 = 0, Uo = 0, lo2 = 0, kbo = 0;
+    if (SHAK && shak) {
+        const double q = 0.5 * tot[NRAW];
+        pre = T(0.8 * (1.0 - S.Pr) * (po[2] * po[2]) / po[0] * q);
+        Ao = T(po[0] * sqrt(po[2] / kPi));
+        nlo2 = T(-po[2] * kLog2e);
+        Uo = T(po[1]);
+        lo2 = T(2.0 * po[2]);
+        kbo = T(K / (2.0 * po[2]));
+    }
+    if (wave == 0 && upd) {
+        const size_t r = (size_t)kc * 3;
+        const gptr<double> wout = to_global(S.w[in ^ 1]);
+        const gptr<double> gp = to_global(S.prim);
+#pragma unroll
+        for (int m = 0; m < 3; m++) {
+            wout[r + m] = wn[m];
+            gp[r + m] = pn[m];
+        }
+    }
+    if (rrow && wave == 0) {
+        // residual sums (solver_update.jl:86-88, update_boundary.jl:41-44): interior / boundary cells
+        double r[12];
+        const bool bnd = i == 1 || i == nx;
+#pragma unroll
+        for (int m = 0; m < 3; m++) {
+            const double d2 = upd ? (wn[m] - wo[m]) * (wn[m] - wo[m]) : 0.0, a = upd ? fabs(wn[m]) : 0.0;
+            r[m] = bnd ? 0.0 : d2;
+            r[3 + m] = bnd ? 0.0 : a;
+            r[6 + m] = bnd ? d2 : 0.0;
+            r[9 + m] = bnd ? a : 0.0;
+        }
+        wave_allreduce<12>(r);
+        double y = 0.0;
+#pragma unroll
+        for (int m = 0; m < 12; m++)
+            if (lane == m) y = r[m];
+        if (lane < 12) rrow[lane] = y;
+    }
+
+    // ---- pass 2: relaxation toward M (+ Shakhov) and the store of f^{n+1}
+    static_for<VW / CH>([&](auto cc) {
+        constexpr int c = cc;
+        const int vb = vbeg + c * CH;
+        if (vb >= vend) return;
+        static_for<CH>([&](auto jc) {
+            constexpr int j = jc;
+            const int v = vb + j;
+            const auto& e = vt[v];
+            const T cn = e.u - Un;
+            T MH = An * texp2(nl2 * (cn * cn));
+            T MB = MH * kbn;
+            if (SHAK) {
+                const T co = e.u - Uo, c2 = co * co;
+                const T Mo = Ao * texp2(nlo2 * c2);
+                const T t = pre * co * Mo;
+                if constexpr (NF == 1) {
+                    MH += t * (lo2 * c2 - T(5));
+                } else {
+                    MH += t * (lo2 * c2 + T(K) - T(5));
+                    MB += (t * kbo) * (lo2 * c2 + T(K) - T(3));
+                }
+            }
+            if (upd && v < vend) {
+                fout[(long)v * np + kc] = G[0][c * CH + j] * inv + MH * rTi;
+                if constexpr (NF == 2) fout[((long)nv + v) * np + kc] = G[1][c * CH + j] * inv + MB * rTi;
+            }
+        });
+    });
+}
+
+template <class T, int NF, int FLUX, int LIMK>
+KBG_D void sweep_coll(const Solve1D& S, const WorkItem& wi, int par, int lane, int wave, int NW, double* lds, double* rrow) {
+    if (S.coll == SHAKHOV) sweep1d<T, NF, FLUX, LIMK, true>(S, wi, par, lane, wave, NW, lds, rrow);
+    else sweep1d<T, NF, FLUX, LIMK, false>(S, wi, par, lane, wave, NW, lds, rrow);
+}
+
+// one workgroup per work item; items of one XCD (workgroup ids congruent mod 8) are a contiguous
+// run of items, so neighbouring segments share that XCD's L2
+template <class T, int NF, int FLUX>
+__global__ __launch_bounds__(1024) void k_sweep1d(const Solve1D* __restrict__ solves, const WorkItem* __restrict__ items,
+                                                  int nitems, int par, double* __restrict__ rpart) {
+    extern __shared__ double kbg_sweep_lds[];
+    const int G = gridDim.x, b = blockIdx.x;
+    const int x = b & 7, q8 = G >> 3, r8 = G & 7;
+    const int it = x * q8 + (x < r8 ? x : r8) + (b >> 3);
+    if (it >= nitems) return;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int NW = blockDim.x >> 6;
+    const WorkItem wi = items[it];
+    const Solve1D& S = solves[wi.solve];
+    double* rrow = rpart ? rpart + (size_t)it * 12 : nullptr;
+    if (!S.interp2) sweep_coll<T, NF, FLUX, LIM_NONE>(S, wi, par, lane, wave, NW, kbg_sweep_lds, rrow);
+    else if (S.limiter == VANLEER) sweep_coll<T, NF, FLUX, LIM_VANLEER>(S, wi, par, lane, wave, NW, kbg_sweep_lds, rrow);
+    else sweep_coll<T, NF, FLUX, LIM_GENERIC>(S, wi, par, lane, wave, NW, kbg_sweep_lds, rrow);
+}
+
+template <class T, int NF, int FLUX>
+hipError_t launch_sweep_t(int nw, const Solve1D* d_solves, const WorkItem* d_items, int nitems, int par, double* d_rpart,
+                          hipStream_t st) {
+    constexpr int NR = SweepCfg<T, NF, FLUX>::NR;
+    const size_t lds = nw > 1 ? (size_t)nw * NR * kSweepLanes * sizeof(double) : 0;
+    hipLaunchKernelGGL((k_sweep1d<T, NF, FLUX>), dim3(nitems), dim3(64 * nw), lds, st, d_solves, d_items, nitems, par, d_rpart);
+    return hipGetLastError();
+}
+
+}  // namespace kbg

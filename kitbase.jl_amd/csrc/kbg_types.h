@@ -1,7 +1,10 @@
 // kbg_types.h -- device-visible solve descriptors and launch entry points of the kbgpu kernels.
 //
-// A solve's state lives in HBM for its whole life.  Distributions are velocity-contiguous per cell:
-// f[cell][nf][nv] (h then b for two distributions), stored in the solve's precision (FP32 or FP64).
+// A solve's state lives in HBM for its whole life.  Distributions are stored in the solve's
+// precision (FP32 or FP64) with explicit strides: element (cell k, distribution q, velocity v) of f
+// is at k fcs + q fqs + v fvs.  Solves stepped by the fused sweep (1D, one velocity dimension) are
+// cell-fastest, f[q][v][np] with each row padded to np cells; the others are velocity-contiguous
+// per cell, f[cell][nf][nv] (h then b).  Slopes sf and face fluxes ff are always [cell|face][nf][nv].
 // Conserved and primitive variables are FP64 [cell][nw].  1D cells are indexed by storage
 // k = i - 1 + ng for the physical 1-based index i in [1-ng, nx+ng]; 1D faces by k - 1 for the
 // physical face k in [1, nx+1] (between cells k-1 and k).  2D cells (i, j) by (i-1) + nx (j-1);
@@ -23,6 +26,9 @@ struct Solve1D {
     int faceL_buf, faceR_buf; // physical face 1 / nx+1 comes from the boundary-face buffers
     int wallL, wallR;         // Maxwell wall at face 1 / nx+1 (2F only)
     int cur;                  // index of the current f / w buffer
+    int swept;                // f is cell-fastest and stepped by k_sweep1d
+    long np;                  // padded cells per velocity row (swept layout)
+    long fcs, fqs, fvs;       // strides of f: cell, distribution, velocity
     double K, gam, mu, om, Pr, dt;
     const double* dx;                              // [nx + 2 ng]
     const void* u;                                 // [nv] velocity nodes (storage type)
@@ -39,6 +45,7 @@ struct Solve1D {
     double* bfw;                                   // [2][nw]  faces 1 and nx+1 (fused path)
     void* bff;                                     // [2][nf][nv]
     const double* bcprim;                          // [2][nw]  wall primitive vectors
+    const void* vtab;                              // [nv + 8] per-velocity constants (swept path)
 };
 
 struct Solve2D {
@@ -66,16 +73,20 @@ struct Solve2D {
     void* yff;
 };
 
-// One unit of work of the fused 1D kernel: cells [c0, c1] (physical) of solve `solve`, marched by
-// one wavefront whose lanes hold vpt velocities each.
+// One unit of work of the fused 1D sweep: cells [c0, c1] (physical, at most 60) of solve `solve`,
+// updated by one workgroup of nw wavefronts that split the velocity grid.
 struct WorkItem {
-    int solve, c0, c1, vpt;
+    int solve, c0, c1, nw;
 };
 
-// ---- launchers (kbg_march.hip, kbg_1d.hip, kbg_2d.hip) --------------------------------------------------------
+// ---- launchers (kbg_sweep.h, kbg_1d.hip, kbg_2d.hip) --------------------------------------------------------
 // All return hipSuccess or the launch error; `fp64` selects the storage type.
-hipError_t launch_march1d(bool fp64, int nf, int flux, int vpt, const Solve1D* d_solves, const WorkItem* d_items,
-                          int nitems, int grid, int par, double* d_rpart, int seg, hipStream_t st);
+hipError_t launch_sweep1d(bool fp64, int nf, int flux, int nw, const Solve1D* d_solves, const WorkItem* d_items,
+                          int nitems, int par, double* d_rpart, hipStream_t st);
+// velocities per wavefront of the sweep for a storage type / distribution count, and the largest
+// velocity grid it takes (16 wavefronts)
+constexpr int sweep_vw(bool fp64, int nf) { return (fp64 ? 16 : 32) / nf; }
+constexpr int sweep_max_nv(bool fp64, int nf) { return 16 * sweep_vw(fp64, nf); }
 hipError_t launch_recon1d(bool fp64, const Solve1D& S, const Solve1D* d_S, hipStream_t st);
 hipError_t launch_evolve1d(bool fp64, const Solve1D& S, const Solve1D* d_S, hipStream_t st);
 hipError_t launch_wall1d(bool fp64, const Solve1D& S, const Solve1D* d_S, int fused, int par, hipStream_t st);

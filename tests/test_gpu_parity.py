@@ -96,7 +96,7 @@ def test_1d_first_order_fp64(gpu_ctx):
     assert rel_max(cg.f, co.f) <= 1e-9
 
 
-@pytest.mark.parametrize("seg", [1, 3, 7, 64, 252])
+@pytest.mark.parametrize("seg", [1, 3, 7, 31, 60])
 def test_fused_segmentation_invariance(gpu_ctx, seg):
     """Work-item length must not change results (halo rows, carried faces, block tails)."""
     old = gpu_ctx.schedule()
@@ -108,12 +108,22 @@ def test_fused_segmentation_invariance(gpu_ctx, seg):
         gpu_ctx.set_schedule(*old)
 
 
-@pytest.mark.parametrize("nu", [32, 64, 65, 128, 150, 256, 300])
+@pytest.mark.parametrize("nu", [24, 32, 33, 64, 65, 101, 128, 150, 256, 300, 512, 513])
 def test_velocity_counts_fp32(gpu_ctx, nu):
-    """Every lane mapping (VPT 1..4) and the phase-exact fallback above 256 velocities."""
+    """Every wavefront count of the sweep (1..10 wavefronts of 32 velocities), partial chunks, and
+    velocity grids that straddle u = 0 inside a chunk."""
     cfg = kb.configs.sod_1d1f1v(nu=nu, vMeshType="rectangle", nx=80, collision="shakhov", knudsen=1e-3)
     ks, cg, co = run_both(cfg, 30, "fp32", ctx=gpu_ctx)
     check_macro(cg.prim, co.prim, 1e-4)
+
+
+@pytest.mark.parametrize("nu", [7, 16, 17, 100, 256, 257])
+def test_velocity_counts_fp64(gpu_ctx, nu):
+    """FP64 sweep (16 velocities per wavefront, up to 256) and the phase-exact path above it."""
+    cfg = kb.configs.sod_1d1f1v(nu=nu, vMeshType="rectangle", nx=70, collision="shakhov", knudsen=1e-3)
+    ks, cg, co = run_both(cfg, 12, "fp64", ctx=gpu_ctx)
+    assert rel_max(cg.f, co.f) <= 1e-9, rel_max(cg.f, co.f)
+    assert rel_max(cg.w, co.w) <= 1e-9
 
 
 def test_phase_exact_1d_matches_oracle_phases(gpu_ctx):
