@@ -184,22 +184,27 @@ int get_f1d(kbg_solve* s, double* dst, const void* src) {
     return KBG_OK;
 }
 
-// per-velocity constants of the sweep for time step dt (kbg_sweep.h VEnt)
+// per-velocity constants of the sweep for time step dt, in chunks of 8 (kbg_sweep.h VChunk)
+template <class T> size_t vtab_bytes(int nv) { return sizeof(kbg::VChunk<T>) * kbg::vchunks(nv); }
 template <class T> void build_vtab(const kbg_desc& d, double dt, std::vector<char>& out) {
-    const int n = d.nv + 8;
-    out.assign(sizeof(kbg::VEnt<T>) * n, 0);
-    auto* e = reinterpret_cast<kbg::VEnt<T>*>(out.data());
-    for (int v = 0; v < n; v++) {
-        e[v].up = 1;
-        if (v >= d.nv) continue;
+    const int nc = kbg::vchunks(d.nv);
+    out.assign(vtab_bytes<T>(d.nv), 0);
+    auto* ch = reinterpret_cast<kbg::VChunk<T>*>(out.data());
+    for (int v = 0; v < nc * kbg::kVChunk; v++) {
+        kbg::VChunk<T>& c = ch[v / kbg::kVChunk];
+        const int j = v % kbg::kVChunk;
+        if (v >= d.nv) {
+            c.upmask |= 1 << j;
+            continue;
+        }
         const double u = d.u[v], w = d.weights[v];
         double p = w;
-        for (int m = 0; m < 5; m++, p *= u) e[v].wk[m] = T(p);
-        e[v].u = T(u);
-        e[v].uu = T(u * u);
-        e[v].cu = T(dt * u);
-        e[v].cs = T(-0.5 * dt * dt * (u * u));
-        e[v].up = u >= 0.0 ? 1 : 0;
+        for (int m = 0; m < 5; m++, p *= u) c.wk[m][j] = T(p);
+        c.u[j] = T(u);
+        c.uu[j] = T(u * u);
+        c.cu[j] = T(dt * u);
+        c.cs[j] = T(-0.5 * dt * dt * (u * u));
+        if (u >= 0.0) c.upmask |= 1 << j;
     }
 }
 
@@ -259,6 +264,14 @@ int validate(const kbg_desc* d) {
 
 int nw_of(const kbg_desc* d) { return d->dim == 2 ? 4 : (d->vdim == 3 ? 5 : 3); }
 
+// the fused sweep takes one velocity dimension, up to 16 wavefronts of velocities, and f buffers
+// addressed with 31-bit byte offsets
+bool swept_1d(const kbg_desc& d) {
+    const size_t N = size_t(d.nx) + 2 * size_t(d.ng), np = (N + 31) / 32 * 32;
+    const size_t bytes = np * d.nf * d.nv * (d.precision == KBG_FP64 ? 8 : 4);
+    return d.dim == 1 && d.vdim == 1 && d.nv <= kbg::sweep_max_nv(d.precision == KBG_FP64, d.nf) && bytes < (size_t(1) << 31);
+}
+
 void fill1d(kbg_solve* s) {
     const kbg_desc& d = s->d;
     Solve1D& S = s->h1;
@@ -296,7 +309,7 @@ void fill1d(kbg_solve* s) {
     S.dt = 0.0;
     // the fused sweep takes one velocity dimension and up to 16 wavefronts of velocities
     const int N = d.nx + 2 * d.ng;
-    S.swept = d.vdim == 1 && d.nv <= kbg::sweep_max_nv(d.precision == KBG_FP64, d.nf);
+    S.swept = swept_1d(d);
     if (S.swept) {
         S.np = (N + 31) / 32 * 32;
         S.fcs = 1;
@@ -500,9 +513,9 @@ size_t kbg_solve_bytes(const kbg_desc* d) {
     const int nw = nw_of(d);
     if (d->dim == 1) {
         const size_t N = size_t(d->nx) + 2 * size_t(d->ng);
-        const bool swept = d->vdim == 1 && d->nv <= kbg::sweep_max_nv(d->precision == KBG_FP64, d->nf);
+        const bool swept = swept_1d(*d);
         const size_t np = swept ? (N + 31) / 32 * 32 : N;
-        const size_t vt = swept ? (size_t(d->nv) + 8) * (esz == 8 ? sizeof(kbg::VEnt<double>) : sizeof(kbg::VEnt<float>)) : 0;
+        const size_t vt = swept ? (esz == 8 ? vtab_bytes<double>(d->nv) : vtab_bytes<float>(d->nv)) : 0;
         return 2 * np * d->nf * d->nv * esz + 3 * N * nw * 8 + N * 8 + 4 * size_t(d->nv) * esz + 2 * d->nf * d->nv * esz +
                N * 4 * nw * 8 + vt;
     }
@@ -585,7 +598,7 @@ int kbg_solve_create(kbg_ctx* ctx, const kbg_desc* desc, kbg_solve** out) {
         s->fused = S.swept != 0;
         if (s->fused) {
             s->sweep_nw = (d.nv + kbg::sweep_vw(s->fp64, d.nf) - 1) / kbg::sweep_vw(s->fp64, d.nf);
-            const size_t vb = (size_t(d.nv) + 8) * (s->fp64 ? sizeof(kbg::VEnt<double>) : sizeof(kbg::VEnt<float>));
+            const size_t vb = s->fp64 ? vtab_bytes<double>(d.nv) : vtab_bytes<float>(d.nv);
             if ((rc = dev_alloc(s, &s->vtab, vb))) return bail(rc);
             HIPCHK(hipMemset(s->vtab, 0, vb));
             S.vtab = s->vtab;

@@ -40,22 +40,33 @@ constexpr int kSweepOut = 60;    // cells a segment updates (lanes 2 .. 61)
 
 // DPP whole-wavefront shifts: shr1(x)[L] = x[L - 1], shl1(x)[L] = x[L + 1] (0 beyond the ends)
 KBG_D float shr1(float x) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x138, 0xf, 0xf, false));
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, x), 0x138, 0xf, 0xf, true));
 }
 KBG_D float shl1(float x) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x130, 0xf, 0xf, false));
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, x), 0x130, 0xf, 0xf, true));
 }
 KBG_D double shr1(double x) {
     const long long b = __builtin_bit_cast(long long, x);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, 0x138, 0xf, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x138, 0xf, 0xf, false);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, 0x138, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0x138, 0xf, 0xf, true);
     return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);
 }
 KBG_D double shl1(double x) {
     const long long b = __builtin_bit_cast(long long, x);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, 0x130, 0xf, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x130, 0xf, 0xf, false);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, 0x130, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0x130, 0xf, 0xf, true);
     return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Raw buffer access to a solve's f: lane byte offset + uniform (scalar) row offset.  Offsets past
+// the buffer read 0 and drop stores.  Word 3 of the resource is the GFX9 raw 32-bit data format.
+using rsrc_t = __amdgpu_buffer_rsrc_t;
+KBG_D rsrc_t make_rsrc(const void* p, unsigned bytes) { return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)bytes, 0x00020000); }
+KBG_D float bload(float*, rsrc_t r, int voff, int soff) { return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0)); }
+KBG_D double bload(double*, rsrc_t r, int voff, int soff) { return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0)); }
+KBG_D void bstore(float x, rsrc_t r, int voff, int soff) { __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), r, voff, soff, 0); }
+KBG_D void bstore(double x, rsrc_t r, int voff, int soff) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, x), r, voff, soff, 0);
 }
 
 // exp2 in the storage precision (the FP32 path uses the hardware exp2 directly)
@@ -67,21 +78,22 @@ KBG_D double trcp(double x) { return 1.0 / x; }
 
 constexpr double kLog2e = 1.4426950408889634074;
 
-// Per-velocity constants of a swept solve, rebuilt on the host when dt changes.  Read through the
-// scalar cache (the velocity index is uniform across a wavefront).  Entries past nv are zero (with
-// up = 1), so padded velocities add nothing to the sums and are never stored.
-template <class T> struct VEnt {
-    T wk[5];   // quadrature weight x u^k, k = 0..4
-    T u, uu;   // u, u^2
-    T cu, cs;  // dt u, -dt^2 u^2 / 2 (KFVS)
-    int up;    // u >= 0: the flux takes the left state (heaviside, math_general.jl:13)
-    int pad_[2];
+// Per-velocity constants of a swept solve in chunks of 8 velocities (structure of arrays, so a
+// chunk's values of one kind arrive in one scalar load), rebuilt on the host when dt changes.
+// Velocities past nv have zero weights and up = 1: they add nothing and are never stored.
+constexpr int kVChunk = 8;
+template <class T> struct alignas(64) VChunk {
+    T wk[5][kVChunk];         // quadrature weight x u^k, k = 0..4
+    T u[kVChunk], uu[kVChunk];
+    T cu[kVChunk], cs[kVChunk];  // dt u, -dt^2 u^2 / 2 (KFVS)
+    int upmask;               // bit j: u >= 0, the flux takes the left state (heaviside, math_general.jl:13)
 };
+constexpr int vchunks(int nv) { return nv / kVChunk + 2; }
 
 // velocities per wavefront and moment-sum counts of a kernel instance
 template <class T, int NF, int FLUX> struct SweepCfg {
     static constexpr int VW = (sizeof(T) == 4 ? 32 : 16) / NF;
-    static constexpr int CH = VW < 8 ? VW : 8;  // velocities loaded together
+    static constexpr int CH = kVChunk;  // velocities loaded together (one table chunk)
     static constexpr int NRAW = FLUX == UGKS ? 6 : (NF == 2 ? 4 : 3);
     static constexpr int NR = NRAW + 1;  // + the Shakhov heat-flux sum
 };
@@ -118,10 +130,14 @@ KBG_D void sweep1d(const Solve1D& S, const WorkItem& wi, int par, int lane, int 
     const int nx = S.nx, ng = S.ng, nv = S.nv, N = nx + 2 * ng;
     const long np = S.np;
     const int in = (S.cur + par) & 1;
-    const gptr<const T> fin = to_global((const T*)S.f[in]);
-    const gptr<T> fout = to_global((T*)S.f[in ^ 1]);
+    // f rows as raw buffers: lane offset k * sizeof(T), row offset (q nv + v) np * sizeof(T)
+    const unsigned fbytes = (unsigned)((long)NF * nv * np * sizeof(T));
+    const rsrc_t fin = make_rsrc(S.f[in], fbytes), fout = make_rsrc(S.f[in ^ 1], fbytes);
+    const int rowb = (int)(np * sizeof(T));
     // read-only for the kernel's lifetime: the constant address space gives scalar (SMEM) loads
-    const __attribute__((address_space(4))) VEnt<T>* vt = (const __attribute__((address_space(4))) VEnt<T>*)S.vtab;
+    // read-only for the kernel's lifetime: the constant address space gives scalar (SMEM) loads
+    using VC = const __attribute__((address_space(4))) VChunk<T>;
+    VC* vt = (VC*)S.vtab;
     const double dt = S.dt, K = S.K;
     const int lk = S.limiter;
 
@@ -130,6 +146,7 @@ KBG_D void sweep1d(const Solve1D& S, const WorkItem& wi, int par, int lane, int 
     const int k = k0 + lane;
     const int kc = k < 0 ? 0 : (k >= N ? N - 1 : k);  // clamped: lanes past the ends load valid memory
     const int i = k - ng + 1;                          // physical index
+    const int voff = k * (int)sizeof(T);               // f lane offset (k < 0 reads 0)
     const double d0 = S.dx[kc];
     const double dm = S.dx[kc > 0 ? kc - 1 : 0], dp = S.dx[kc < N - 1 ? kc + 1 : N - 1];
     const bool slope_on = S.interp2 && i >= S.slope_lo && i <= S.slope_hi && k >= 0 && k < N;
@@ -220,25 +237,34 @@ KBG_D void sweep1d(const Solve1D& S, const WorkItem& wi, int par, int lane, int 
     const gptr<const T> bff = to_global((const T*)S.bff);
     const T UoT = T(Uo64);
 
+    // f rows of chunk c into buffer c & 1 (the next chunk's loads are in flight while a chunk computes)
+    T frb[2][NF][CH];
+    auto load_chunk = [&](auto cc) {
+        constexpr int c = cc;
+        const int vb = vbeg + c * CH;
+        if (vb >= vend) return;
+        static_for<CH>([&](auto jc) {
+            constexpr int j = jc;
+#pragma unroll
+            for (int q = 0; q < NF; q++) frb[c & 1][q][j] = bload((T*)nullptr, fin, voff, (q * nv + vb + j) * rowb);
+        });
+    };
+
     // one chunk of CH velocities; UPM: 0 all take the right state, 1 all the left, 2 mixed;
     // BUF: a lane's left face comes from the boundary-face buffers
     auto chunk1 = [&](auto cc, auto upm, auto buf) {
         constexpr int c = cc, UPM = upm;
         constexpr bool BUF = buf;
         const int vb = vbeg + c * CH;
-        T fr[NF][CH];
-        static_for<CH>([&](auto jc) {
-            constexpr int j = jc;
-            const int v = min(vb + j, nv - 1);
-#pragma unroll
-            for (int q = 0; q < NF; q++) fr[q][j] = fin[((long)q * nv + v) * np + kc];
-        });
+        VC& ch = vt[vb / CH];
+        T(&fr)[NF][CH] = frb[c & 1];
         T part[NR];  // chunk sums in the storage precision, added to the FP64 totals below
 #pragma unroll
         for (int m = 0; m < NR; m++) part[m] = T(0);
         static_for<CH>([&](auto jc) {
             constexpr int j = jc;
-            const auto& e = vt[vb + j];
+            const T u = ch.u[j], uu = ch.uu[j];
+            const bool up = (ch.upmask >> j) & 1;
             T fq[NF], sq[NF], fu[NF], su[NF], ffq[NF];
 #pragma unroll
             for (int q = 0; q < NF; q++) {
@@ -258,22 +284,22 @@ KBG_D void sweep1d(const Solve1D& S, const WorkItem& wi, int par, int lane, int 
                 } else {
                     const T l = shr1(fq[q] + sq[q] * dxh), r = fq[q] - sq[q] * dxh;
                     const T sl = shr1(sq[q]);
-                    fu[q] = e.up ? l : r;
-                    su[q] = e.up ? sl : sq[q];
+                    fu[q] = up ? l : r;
+                    su[q] = up ? sl : sq[q];
                 }
             }
             if constexpr (FLUX == KFVS) {
 #pragma unroll
-                for (int q = 0; q < NF; q++) ffq[q] = fu[q] * e.cu + su[q] * e.cs;
+                for (int q = 0; q < NF; q++) ffq[q] = fu[q] * ch.cu[j] + su[q] * ch.cs[j];
             } else if constexpr (FLUX == KCU) {
                 // flux_kcu.jl:241-303 (1F1V), :463-530 (2F1V)
-                const T cI = e.u - F.Ui;
+                const T cI = u - F.Ui;
                 const T g = F.A * texp2(F.nl2 * (cI * cI));
-                ffq[0] = F.M0 * e.u * g + F.M1 * e.u * fu[0];
-                if constexpr (NF == 2) ffq[1] = F.M0 * e.u * (g * F.kb) + F.M1 * e.u * fu[1];
+                ffq[0] = F.M0 * u * g + F.M1 * u * fu[0];
+                if constexpr (NF == 2) ffq[1] = F.M0 * u * (g * F.kb) + F.M1 * u * fu[1];
             } else {
                 // flux_ugks.jl:91-111 (1D2F1V)
-                const T x = e.u, uu = e.uu;
+                const T x = u;
                 const T cI = x - F.Ui;
                 const T H = F.A * texp2(F.nl2 * (cI * cI));
                 const T Bq = H * F.kb;
@@ -283,7 +309,7 @@ KBG_D void sweep1d(const Solve1D& S, const WorkItem& wi, int par, int lane, int 
                 } else if constexpr (UPM == 0) {
                     a0 = F.aR0, a1 = F.aR1, a2 = F.aR2;
                 } else {
-                    a0 = e.up ? F.aL0 : F.aR0, a1 = e.up ? F.aL1 : F.aR1, a2 = e.up ? F.aL2 : F.aR2;
+                    a0 = up ? F.aL0 : F.aR0, a1 = up ? F.aL1 : F.aR1, a2 = up ? F.aL2 : F.aR2;
                 }
                 const T pH = a0 * H + a1 * x * H + T(0.5) * a2 * (uu * H + Bq);
                 const T pTH = F.aT0 * H + F.aT1 * x * H + T(0.5) * F.aT2 * (uu * H + Bq);
@@ -303,27 +329,27 @@ KBG_D void sweep1d(const Solve1D& S, const WorkItem& wi, int par, int lane, int 
             }
             // moment sums of the face
             if constexpr (FLUX == KFVS) {
-                part[0] += e.wk[0] * ffq[0];
-                part[1] += e.wk[1] * ffq[0];
-                part[2] += e.wk[2] * ffq[0];
-                if constexpr (NF == 2) part[3] += e.wk[0] * ffq[1];
+                part[0] += ch.wk[0][j] * ffq[0];
+                part[1] += ch.wk[1][j] * ffq[0];
+                part[2] += ch.wk[2][j] * ffq[0];
+                if constexpr (NF == 2) part[3] += ch.wk[0][j] * ffq[1];
             } else if constexpr (FLUX == KCU) {
-                part[0] += e.wk[1] * fu[0];
-                part[1] += e.wk[2] * fu[0];
-                part[2] += e.wk[3] * fu[0];
-                if constexpr (NF == 2) part[3] += e.wk[1] * fu[1];
+                part[0] += ch.wk[1][j] * fu[0];
+                part[1] += ch.wk[2][j] * fu[0];
+                part[2] += ch.wk[3][j] * fu[0];
+                if constexpr (NF == 2) part[3] += ch.wk[1][j] * fu[1];
             } else {
-                part[0] += e.wk[1] * fu[0];
-                part[1] += e.wk[2] * su[0];
-                part[2] += e.wk[2] * fu[0];
-                part[3] += e.wk[3] * su[0];
-                part[4] += e.wk[3] * fu[0] + e.wk[1] * fu[1];
-                part[5] += e.wk[4] * su[0] + e.wk[2] * su[1];
+                part[0] += ch.wk[1][j] * fu[0];
+                part[1] += ch.wk[2][j] * su[0];
+                part[2] += ch.wk[2][j] * fu[0];
+                part[3] += ch.wk[3][j] * su[0];
+                part[4] += ch.wk[3][j] * fu[0] + ch.wk[1][j] * fu[1];
+                part[5] += ch.wk[4][j] * su[0] + ch.wk[2][j] * su[1];
             }
             if constexpr (SHAK) {
                 // heat flux of the cell from the old f and prim (theory_moments.jl:523, :530-531)
-                const T co = e.u - UoT;
-                const T wc = e.wk[0] * co;
+                const T co = u - UoT;
+                const T wc = ch.wk[0][j] * co;
                 part[NRAW] += (wc * (co * co)) * fq[0];
                 if constexpr (NF == 2) part[NRAW] += wc * fq[1];
             }
@@ -339,15 +365,17 @@ KBG_D void sweep1d(const Solve1D& S, const WorkItem& wi, int par, int lane, int 
     using I2 = std::integral_constant<int, 2>;
     using BT = std::integral_constant<bool, true>;
     using BF = std::integral_constant<bool, false>;
+    load_chunk(I0{});
     static_for<VW / CH>([&](auto cc) {
         constexpr int c = cc;
         const int vb = vbeg + c * CH;
         if (vb >= vend) return;
+        if constexpr (c + 1 < VW / CH) load_chunk(std::integral_constant<int, c + 1>{});
         if (LIMK == LIM_VANLEER && !seg_buf) {
-            // velocities are sorted: a chunk whose ends agree in sign takes one side throughout
-            const int u0 = vt[vb].up, u1 = vt[min(vb + CH - 1, nv - 1)].up;
-            if (u0 && u1) chunk1(cc, I1{}, BF{});
-            else if (!u0 && !u1) chunk1(cc, I0{}, BF{});
+            // velocities are sorted: a chunk whose velocities agree in sign takes one side throughout
+            const int m = vt[vb / CH].upmask;
+            if (m == 0xff) chunk1(cc, I1{}, BF{});
+            else if (m == 0) chunk1(cc, I0{}, BF{});
             else chunk1(cc, I2{}, BF{});
         } else if (seg_buf) {
             chunk1(cc, I2{}, BT{});
@@ -447,15 +475,16 @@ KBG_D void sweep1d(const Solve1D& S, const WorkItem& wi, int par, int lane, int 
         constexpr int c = cc;
         const int vb = vbeg + c * CH;
         if (vb >= vend) return;
+        VC& ch = vt[vb / CH];
+        T fn[NF][CH];
         static_for<CH>([&](auto jc) {
             constexpr int j = jc;
-            const int v = vb + j;
-            const auto& e = vt[v];
-            const T cn = e.u - Un;
+            const T u = ch.u[j];
+            const T cn = u - Un;
             T MH = An * texp2(nl2 * (cn * cn));
             T MB = MH * kbn;
-            if (SHAK) {
-                const T co = e.u - Uo, c2 = co * co;
+            if constexpr (SHAK) {
+                const T co = u - Uo, c2 = co * co;
                 const T Mo = Ao * texp2(nlo2 * c2);
                 const T t = pre * co * Mo;
                 if constexpr (NF == 1) {
@@ -465,11 +494,26 @@ KBG_D void sweep1d(const Solve1D& S, const WorkItem& wi, int par, int lane, int 
                     MB += (t * kbo) * (lo2 * c2 + T(K) - T(3));
                 }
             }
-            if (upd && v < vend) {
-                fout[(long)v * np + kc] = G[0][c * CH + j] * inv + MH * rTi;
-                if constexpr (NF == 2) fout[((long)nv + v) * np + kc] = G[1][c * CH + j] * inv + MB * rTi;
-            }
+            fn[0][j] = G[0][c * CH + j] * inv + MH * rTi;
+            if constexpr (NF == 2) fn[1][j] = G[1][c * CH + j] * inv + MB * rTi;
         });
+        if (upd) {
+            if (vb + CH <= vend) {
+                static_for<CH>([&](auto jc) {
+                    constexpr int j = jc;
+#pragma unroll
+                    for (int q = 0; q < NF; q++) bstore(fn[q][j], fout, voff, (q * nv + vb + j) * rowb);
+                });
+            } else {
+                static_for<CH>([&](auto jc) {
+                    constexpr int j = jc;
+                    if (vb + j < vend) {
+#pragma unroll
+                        for (int q = 0; q < NF; q++) bstore(fn[q][j], fout, voff, (q * nv + vb + j) * rowb);
+                    }
+                });
+            }
+        }
     });
 }
 
