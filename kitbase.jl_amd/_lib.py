@@ -87,7 +87,7 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
     global _lib
     if _lib is not None and path is None:
         return _lib
-    p = Path(path) if path else LIB_PATH
+    p = Path(path) if path else Path(os.environ.get("KBG_LIBRARY", LIB_PATH))
     if not p.exists():
         raise KbgError(f"{p} not found: build it with `python kitbase.jl_amd/build.py`")
     lib = C.CDLL(str(p))

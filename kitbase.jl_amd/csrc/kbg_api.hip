@@ -264,7 +264,7 @@ int validate(const kbg_desc* d) {
 
 int nw_of(const kbg_desc* d) { return d->dim == 2 ? 4 : (d->vdim == 3 ? 5 : 3); }
 
-// the fused sweep takes one velocity dimension, up to 16 wavefronts of velocities, and f buffers
+// the fused sweep takes one velocity dimension, up to kSweepMaxWaves wavefronts of velocities, and f buffers
 // addressed with 31-bit byte offsets
 bool swept_1d(const kbg_desc& d) {
     const size_t N = size_t(d.nx) + 2 * size_t(d.ng), np = (N + 31) / 32 * 32;
@@ -307,7 +307,7 @@ void fill1d(kbg_solve* s) {
     S.om = d.omega;
     S.Pr = d.Pr;
     S.dt = 0.0;
-    // the fused sweep takes one velocity dimension and up to 16 wavefronts of velocities
+    // the fused sweep takes one velocity dimension and up to kSweepMaxWaves wavefronts of velocities
     const int N = d.nx + 2 * d.ng;
     S.swept = swept_1d(d);
     if (S.swept) {

@@ -92,7 +92,7 @@ constexpr int vchunks(int nv) { return nv / kVChunk + 2; }
 
 // velocities per wavefront and moment-sum counts of a kernel instance
 template <class T, int NF, int FLUX> struct SweepCfg {
-    static constexpr int VW = (sizeof(T) == 4 ? 32 : 16) / NF;
+    static constexpr int VW = sweep_vw(sizeof(T) == 8, NF);
     static constexpr int CH = kVChunk;  // velocities loaded together (one table chunk)
     static constexpr int NRAW = FLUX == UGKS ? 6 : (NF == 2 ? 4 : 3);
     static constexpr int NR = NRAW + 1;  // + the Shakhov heat-flux sum
@@ -160,23 +160,19 @@ KBG_D void sweep1d(const Solve1D& S, const WorkItem& wi, int par, int lane, int 
     const bool bufL = (i == 1 && S.faceL_buf && S.upd_lo == 1) || (i == nx + 1 && S.faceR_buf && S.upd_hi == nx);
     const int bside = i == 1 ? 0 : 1;
     const bool seg_buf = __builtin_amdgcn_readfirstlane(__ballot(bufL) != 0ull ? 1 : 0) != 0;
-    double wo[3];
+    // the cell's w and prim are read where they are used (after pass 1), to keep pass 1's
+    // register footprint small; pass 1 needs only the old velocity for the Shakhov heat flux
     const gptr<const double> win = to_global((const double*)S.w[in]);
-#pragma unroll
-    for (int m = 0; m < 3; m++) wo[m] = win[(size_t)kc * 3 + m];
     const bool shak = SHAK && i > 1 && i < nx;  // boundary cells relax with BGK (update_boundary.jl)
-    double po[3] = {1.0, 0.0, 1.0};
-    if (SHAK) {
-#pragma unroll
-        for (int m = 0; m < 3; m++) po[m] = S.prim[(size_t)kc * 3 + m];
-    }
-    const double Uo64 = po[1];
+    const double Uo64 = SHAK ? S.prim[(size_t)kc * 3 + 1] : 0.0;
 
     // ---- face coefficients of the lane's left face (KCU / UGKS), FP64 per lane
     SFace<T> F{};
     double fw0[3] = {0, 0, 0}, m3 = 0.0, m4 = 0.0;
     if constexpr (FLUX != KFVS) {
-        double sw[3], wL[3], wR[3];
+        double wo[3], sw[3], wL[3], wR[3];
+#pragma unroll
+        for (int m = 0; m < 3; m++) wo[m] = win[(size_t)kc * 3 + m];
         const double dxL = shr1(dxh64);
 #pragma unroll
         for (int m = 0; m < 3; m++) {
@@ -401,6 +397,14 @@ KBG_D void sweep1d(const Solve1D& S, const WorkItem& wi, int par, int lane, int 
         for (int m = 0; m < NR; m++) tot[m] = acc[m];
     }
 
+    double wo[3], po[3] = {1.0, 0.0, 1.0};
+#pragma unroll
+    for (int m = 0; m < 3; m++) wo[m] = win[(size_t)kc * 3 + m];
+    if (SHAK) {
+#pragma unroll
+        for (int m = 0; m < 3; m++) po[m] = S.prim[(size_t)kc * 3 + m];
+    }
+
     // ---- face moments of the lane's left face, and of its right face (the next lane's left)
     double fwl[3];
     if constexpr (FLUX == KFVS) {
@@ -525,8 +529,12 @@ KBG_D void sweep_coll(const Solve1D& S, const WorkItem& wi, int par, int lane, i
 
 // one workgroup per work item; items of one XCD (workgroup ids congruent mod 8) are a contiguous
 // run of items, so neighbouring segments share that XCD's L2
+// occupancy target: 4 wavefronts per SIMD (<= 128 VGPRs) for the FP32 one-distribution sweeps,
+// whatever the register file allows for the others
+template <class T, int NF, int FLUX> constexpr int sweep_waves_per_eu() { return sizeof(T) == 4 && NF == 1 ? 4 : 1; }
+
 template <class T, int NF, int FLUX>
-__global__ __launch_bounds__(1024) void k_sweep1d(const Solve1D* __restrict__ solves, const WorkItem* __restrict__ items,
+__global__ __launch_bounds__(64 * kSweepMaxWaves) __attribute__((amdgpu_waves_per_eu(sweep_waves_per_eu<T, NF, FLUX>()))) void k_sweep1d(const Solve1D* __restrict__ solves, const WorkItem* __restrict__ items,
                                                   int nitems, int par, double* __restrict__ rpart) {
     extern __shared__ double kbg_sweep_lds[];
     const int G = gridDim.x, b = blockIdx.x;

@@ -83,10 +83,15 @@ struct WorkItem {
 // All return hipSuccess or the launch error; `fp64` selects the storage type.
 hipError_t launch_sweep1d(bool fp64, int nf, int flux, int nw, const Solve1D* d_solves, const WorkItem* d_items,
                           int nitems, int par, double* d_rpart, hipStream_t st);
-// velocities per wavefront of the sweep for a storage type / distribution count, and the largest
-// velocity grid it takes (16 wavefronts)
-constexpr int sweep_vw(bool fp64, int nf) { return (fp64 ? 16 : 32) / nf; }
-constexpr int sweep_max_nv(bool fp64, int nf) { return 16 * sweep_vw(fp64, nf); }
+// velocities per wavefront of the sweep for a storage type / distribution count (a multiple of 8),
+// and the largest velocity grid it takes (8 wavefronts)
+#ifndef KBG_SWEEP_VW32
+#define KBG_SWEEP_VW32 64
+#endif
+constexpr int kSweepMaxWaves = 8;
+constexpr int sweep_vw_raw(bool fp64, int nf) { return (fp64 ? KBG_SWEEP_VW32 / 2 : KBG_SWEEP_VW32) / nf; }
+constexpr int sweep_vw(bool fp64, int nf) { return sweep_vw_raw(fp64, nf) < 8 ? 8 : sweep_vw_raw(fp64, nf); }
+constexpr int sweep_max_nv(bool fp64, int nf) { return kSweepMaxWaves * sweep_vw(fp64, nf); }
 hipError_t launch_recon1d(bool fp64, const Solve1D& S, const Solve1D* d_S, hipStream_t st);
 hipError_t launch_evolve1d(bool fp64, const Solve1D& S, const Solve1D* d_S, hipStream_t st);
 hipError_t launch_wall1d(bool fp64, const Solve1D& S, const Solve1D* d_S, int fused, int par, hipStream_t st);
