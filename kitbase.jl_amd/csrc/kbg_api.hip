@@ -57,6 +57,8 @@ struct kbg_ctx {
     size_t items_cap = 0;
     double* d_rpart = nullptr;
     size_t rpart_cap = 0;
+    kbg::Pack* d_packs = nullptr;
+    size_t packs_cap = 0;
     double* d_res = nullptr;  // [4096][5]
 };
 
@@ -204,7 +206,17 @@ template <class T> void build_vtab(const kbg_desc& d, double dt, std::vector<cha
         c.uu[j] = T(u * u);
         c.cu[j] = T(dt * u);
         c.cs[j] = T(-0.5 * dt * dt * (u * u));
+        c.cs2[j] = T(-dt * dt * (u * u));
         if (u >= 0.0) c.upmask |= 1 << j;
+    }
+    // equally spaced chunks (to 1e-12 relative) carry their node spacing
+    for (int c = 0; c < nc; c++) {
+        const int v0 = c * kbg::kVChunk, v1 = std::min(d.nv, v0 + kbg::kVChunk) - 1;
+        if (v1 - v0 < 1) continue;
+        const double du = (d.u[v1] - d.u[v0]) / (v1 - v0);
+        bool eq = du > 0.0;
+        for (int v = v0; v < v1 && eq; v++) eq = std::fabs(d.u[v + 1] - d.u[v] - du) <= 1e-12 * du;
+        if (eq) ch[c].du = T(du);
     }
 }
 
@@ -461,6 +473,7 @@ int kbg_ctx_destroy(kbg_ctx* c) {
     (void)hipFree(c->d_descs);
     (void)hipFree(c->d_items);
     (void)hipFree(c->d_rpart);
+    (void)hipFree(c->d_packs);
     (void)hipFree(c->d_res);
     for (auto& sd : c->side)
         if (sd) (void)hipStreamDestroy(sd);
@@ -597,7 +610,7 @@ int kbg_solve_create(kbg_ctx* ctx, const kbg_desc* desc, kbg_solve** out) {
         s->d1 = (Solve1D*)p;
         s->fused = S.swept != 0;
         if (s->fused) {
-            s->sweep_nw = (d.nv + kbg::sweep_vw(s->fp64, d.nf) - 1) / kbg::sweep_vw(s->fp64, d.nf);
+            s->sweep_nw = kbg::sweep_nw(s->fp64, d.nf, d.nv);
             const size_t vb = s->fp64 ? vtab_bytes<double>(d.nv) : vtab_bytes<float>(d.nv);
             if ((rc = dev_alloc(s, &s->vtab, vb))) return bail(rc);
             HIPCHK(hipMemset(s->vtab, 0, vb));
@@ -848,61 +861,80 @@ int kbg_batch_step(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double* 
     if (nsteps <= 0 || n <= 0) return KBG_OK;
     hipStream_t st = ctx->stream;
     int rc;
-    // fused 1D solves grouped by kernel instance; the rest stepped phase by phase
-    std::map<std::tuple<int, int, int, int>, std::vector<int>> groups;
-    std::vector<int> others;
+    // fused 1D solves go to the sweep; the rest are stepped phase by phase
+    std::vector<int> fused, others;
     for (int i = 0; i < n; i++) {
         kbg_solve* s = solves[i];
         if (!s || s->ctx != ctx) return fail(KBG_ERR_ARG, "solve is NULL or belongs to another context");
         if (s->d.dim == 1 && s->fused) {
-            groups[{s->fp64 ? 1 : 0, s->d.nf, s->d.flux, s->sweep_nw}].push_back(i);
+            fused.push_back(i);
             if ((rc = ensure_vtab(s, dts[i]))) return rc;
         } else {
             others.push_back(i);
         }
     }
-    // descriptors and work items of all fused solves
+    const int seg = std::max(1, ctx->seg_cells);
+    // the fused solves are spread over the streams by work (cells x velocities), heaviest first;
+    // solves are independent, so each stream advances its own share
+    const int nstreams = std::max(1, std::min<int>(ctx->streams > 0 ? ctx->streams : 1 + kbg_ctx::kSide, (int)fused.size()));
+    std::vector<int> stream_of(n, 0);
+    {
+        std::vector<int> order(fused);
+        auto work = [&](int i) { return double(solves[i]->N) * solves[i]->d.nv * solves[i]->d.nf; };
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work(a) > work(b); });
+        std::vector<double> load(nstreams, 0.0);
+        for (int i : order) {
+            const int k = int(std::min_element(load.begin(), load.end()) - load.begin());
+            stream_of[i] = k;
+            load[k] += work(i);
+        }
+    }
+    // descriptors, work items and packs; one launch per (stream, kernel instance)
+    struct Launch {
+        bool fp64;
+        int nf, flux, wg_waves, stream;
+        size_t pack0, npacks;
+        std::vector<int> members;  // solve indices (for the wall kernels)
+    };
     std::vector<Solve1D> descs;
     std::vector<int> desc_of(n, -1);
-    struct Group {
-        bool fp64;
-        int nf, flux, nw;
-        size_t item0, nitems;
-        std::vector<int> members;  // solve indices
-        double work;               // wavefront-segments per step
-        int stream;                // 0: the context stream, k > 0: side stream k - 1
-    };
-    std::vector<Group> gl;
     std::vector<WorkItem> items;
-    const int seg = std::max(1, ctx->seg_cells);
-    for (auto& kv : groups) {
-        Group g{std::get<0>(kv.first) == 1, std::get<1>(kv.first), std::get<2>(kv.first), std::get<3>(kv.first), items.size(), 0,
-                kv.second, 0.0, 0};
-        std::vector<WorkItem> gi;
+    std::vector<kbg::Pack> packs;
+    std::vector<Launch> launches;
+    std::map<std::tuple<int, int, int, int>, std::vector<int>> keyed;  // (stream, fp64, nf, flux) -> solves
+    for (int i : fused) keyed[{stream_of[i], solves[i]->fp64 ? 1 : 0, solves[i]->d.nf, solves[i]->d.flux}].push_back(i);
+    for (auto& kv : keyed) {
+        Launch L{std::get<1>(kv.first) == 1, std::get<2>(kv.first), std::get<3>(kv.first), 1, std::get<0>(kv.first),
+                 packs.size(), 0, kv.second};
+        std::map<int, std::vector<int>> by_nw;  // nw -> item indices, in solve order
         for (int i : kv.second) {
             kbg_solve* s = solves[i];
             s->h1.dt = dts[i];
             desc_of[i] = (int)descs.size();
-            build_items(s, seg, desc_of[i], gi);
             descs.push_back(s->h1);
+            const size_t i0 = items.size();
+            build_items(s, seg, desc_of[i], items);
+            for (size_t it = i0; it < items.size(); it++) by_nw[s->sweep_nw].push_back((int)it);
+            L.wg_waves = std::max(L.wg_waves, s->sweep_nw);
         }
-        g.nitems = gi.size();
-        g.work = double(g.nitems) * g.nw;
-        items.insert(items.end(), gi.begin(), gi.end());
-        gl.push_back(g);
-    }
-    // heaviest groups first, each to the least-loaded stream
-    std::sort(gl.begin(), gl.end(), [](const Group& a, const Group& b) { return a.work > b.work; });
-    {
-        double load[1 + kbg_ctx::kSide] = {};
-        const int ns = std::min<int>(ctx->streams > 0 ? ctx->streams : 1 + kbg_ctx::kSide, (int)gl.size());
-        for (Group& g : gl) {
-            int best = 0;
-            for (int k = 1; k < ns; k++)
-                if (load[k] < load[best]) best = k;
-            g.stream = best;
-            load[best] += g.work;
+        // workgroups of at least 4 wavefronts: side-by-side items of the same nw fill them
+        L.wg_waves = std::max(L.wg_waves, std::min(4, kbg::kSweepMaxWaves));
+        // items are contiguous per nw bucket, so a pack is a run of consecutive items
+        std::vector<WorkItem> reordered;
+        for (auto& b : by_nw) {
+            const int per = L.wg_waves / b.first;
+            for (size_t k = 0; k < b.second.size(); k += per) {
+                kbg::Pack p{(int)(items.size() + reordered.size()), (int)std::min<size_t>(per, b.second.size() - k), b.first, 0};
+                for (int m = 0; m < p.count; m++) reordered.push_back(items[b.second[k + m]]);
+                packs.push_back(p);
+            }
         }
+        // the reordered items replace this launch's items at the end of the list
+        const size_t first = items.size() - reordered.size();
+        for (kbg::Pack* p = packs.data() + L.pack0; p != packs.data() + packs.size(); ++p) p->item0 -= (int)reordered.size();
+        std::copy(reordered.begin(), reordered.end(), items.begin() + first);
+        L.npacks = packs.size() - L.pack0;
+        launches.push_back(std::move(L));
     }
     if (!descs.empty()) {
         if (descs.size() > ctx->descs_cap) {
@@ -917,34 +949,38 @@ int kbg_batch_step(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double* 
             HIPCHK(hipMalloc((void**)&ctx->d_items, ctx->items_cap * sizeof(WorkItem)));
             HIPCHK(hipMalloc((void**)&ctx->d_rpart, ctx->items_cap * 12 * sizeof(double)));
         }
+        if (packs.size() > ctx->packs_cap) {
+            (void)hipFree(ctx->d_packs);
+            ctx->packs_cap = packs.size() * 2;
+            HIPCHK(hipMalloc((void**)&ctx->d_packs, ctx->packs_cap * sizeof(kbg::Pack)));
+        }
         HIPCHK(hipMemcpyAsync(ctx->d_descs, descs.data(), descs.size() * sizeof(Solve1D), hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(ctx->d_items, items.data(), items.size() * sizeof(WorkItem), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(ctx->d_packs, packs.data(), packs.size() * sizeof(kbg::Pack), hipMemcpyHostToDevice, st));
         HIPCHK(hipStreamSynchronize(st));
     }
     HIPCHK(hipEventRecord(ctx->ev[KBG_EV_STEPS_BEGIN], st));
-    // group gi runs on stream gst[gi]; groups share no data, so they only fork from and join
-    // back into the context stream
-    int nside = 0;
-    for (const Group& g : gl) nside = std::max(nside, g.stream);
-    std::vector<hipStream_t> gst(gl.size(), st);
+    // stream k > 0 is side stream k - 1; they fork from and join back into the context stream
+    const int nside = nstreams - 1;
+    std::vector<hipStream_t> sst(nstreams, st);
     if (nside > 0) {
         HIPCHK(hipEventRecord(ctx->fork, st));
-        for (int k = 0; k < nside; k++) HIPCHK(hipStreamWaitEvent(ctx->side[k], ctx->fork, 0));
-        for (size_t gi = 0; gi < gl.size(); gi++) gst[gi] = gl[gi].stream == 0 ? st : ctx->side[gl[gi].stream - 1];
+        for (int k = 0; k < nside; k++) {
+            HIPCHK(hipStreamWaitEvent(ctx->side[k], ctx->fork, 0));
+            sst[k + 1] = ctx->side[k];
+        }
     }
     for (int t = 0; t < nsteps; t++) {
         const int par = t & 1;
         const bool last = t == nsteps - 1;
-        for (size_t gi = 0; gi < gl.size(); gi++) {
-            const Group& g = gl[gi];
-            for (int di : g.members) {
+        for (const Launch& L : launches) {
+            for (int di : L.members) {
                 const Solve1D& S = descs[desc_of[di]];
                 if (S.wallL || S.wallR)
-                    HIPCHK(kbg::launch_wall1d(solves[di]->fp64, S, ctx->d_descs + desc_of[di], 1, par, gst[gi]));
+                    HIPCHK(kbg::launch_wall1d(solves[di]->fp64, S, ctx->d_descs + desc_of[di], 1, par, sst[L.stream]));
             }
-            double* rp = (last && residuals) ? ctx->d_rpart + g.item0 * 12 : nullptr;
-            HIPCHK(kbg::launch_sweep1d(g.fp64, g.nf, g.flux, g.nw, ctx->d_descs, ctx->d_items + g.item0, (int)g.nitems, par,
-                                       rp, gst[gi]));
+            HIPCHK(kbg::launch_sweep1d(L.fp64, L.nf, L.flux, L.wg_waves, ctx->d_descs, ctx->d_items, ctx->d_packs + L.pack0,
+                                       (int)L.npacks, par, (last && residuals) ? ctx->d_rpart : nullptr, sst[L.stream]));
         }
         for (int i : others) {
             kbg_solve* s = solves[i];

@@ -30,6 +30,8 @@
 //   cells     solver_update.jl:61-93, update_boundary.jl:6-61; step_1f.jl:6-55, step_2f.jl:6-64
 #pragma once
 
+#include <algorithm>
+
 #include "kbg_device.h"
 #include "kbg_types.h"
 
@@ -86,6 +88,8 @@ template <class T> struct alignas(64) VChunk {
     T wk[5][kVChunk];         // quadrature weight x u^k, k = 0..4
     T u[kVChunk], uu[kVChunk];
     T cu[kVChunk], cs[kVChunk];  // dt u, -dt^2 u^2 / 2 (KFVS)
+    T cs2[kVChunk];           // -dt^2 u^2 (KFVS with half slopes)
+    T du;                     // node spacing when the chunk's nodes are equally spaced, else 0
     int upmask;               // bit j: u >= 0, the flux takes the left state (heaviside, math_general.jl:13)
 };
 constexpr int vchunks(int nv) { return nv / kVChunk + 2; }
@@ -102,18 +106,19 @@ template <class T, int NF, int FLUX> struct SweepCfg {
 // take the generic runtime switch
 enum { LIM_VANLEER = 0, LIM_GENERIC = 1, LIM_NONE = 2 };
 
-template <class T, int LIMK> KBG_D T sweep_limit(int kind, T sL, T sR) {
+// HALF: return half the slope (the caller doubles the factors it multiplies the slope by)
+template <class T, int LIMK, bool HALF> KBG_D T sweep_limit(int kind, T sL, T sR) {
     if constexpr (LIMK == LIM_NONE) {
         return T(0);
     } else if constexpr (LIMK == LIM_VANLEER) {
         // (sgn sL + sgn sR) |sL| |sR| / (|sL| + |sR| + 1e-7) (recon_limiter.jl:17-19): zero unless
-        // sL sR > 0, and then 2 sL sR / (sL + sR + copysign(1e-7, sL))
-        const T p = sL * sR;
-        const T den = sL + sR + copysign(T(1e-7), sL);
-        const T r = T(2) * p * trcp(den);
-        return p > T(0) ? r : T(0);
+        // sL sR > 0, and then 2 |sL sR| / (|sL| + |sR| + 1e-7) with the sign of sL
+        const T p = fmax(sL * sR, T(0));
+        const T r = p * trcp(fabs(sL) + fabs(sR) + T(1e-7));
+        return copysign(HALF ? r : T(2) * r, sL);
     } else {
-        return limiter<T>(kind, sL, sR);
+        const T r = limiter<T>(kind, sL, sR);
+        return HALF ? T(0.5) * r : r;
     }
 }
 
@@ -155,6 +160,9 @@ KBG_D void sweep1d(const Solve1D& S, const WorkItem& wi, int par, int lane, int 
     const T rL = T(rL64), rR = T(rR64);
     const double dxh64 = 0.5 * d0, rdx64 = 1.0 / d0;
     const T dxh = T(dxh64), rdx = T(rdx64);
+    // KFVS carries half slopes (one multiply less in the limiter): the half cell width doubles
+    constexpr bool HALF = FLUX == KFVS;
+    const T dxs = HALF ? T(2.0 * dxh64) : dxh;
     const bool upd = lane >= 2 && lane < 2 + kSweepOut && i >= wi.c0 && i <= wi.c1;
     // the lane's left face comes from the boundary-face buffers (only where a stepped cell uses it)
     const bool bufL = (i == 1 && S.faceL_buf && S.upd_lo == 1) || (i == nx + 1 && S.faceR_buf && S.upd_hi == nx);
@@ -224,8 +232,9 @@ KBG_D void sweep1d(const Solve1D& S, const WorkItem& wi, int par, int lane, int 
     }
 
     // ---- pass 1: slopes, face fluxes, moment sums, transport update
-    const int vbeg = wave * VW;
-    const int vend = min(nv, vbeg + VW);
+    const int ivw = sweep_item_vw(nv, NW);  // this item's velocities per wavefront (<= VW)
+    const int vbeg = wave * ivw;
+    const int vend = min(nv, vbeg + ivw);
     T G[NF][VW];
     double acc[NR];
 #pragma unroll
@@ -266,19 +275,19 @@ KBG_D void sweep1d(const Solve1D& S, const WorkItem& wi, int par, int lane, int 
             for (int q = 0; q < NF; q++) {
                 const T f = fr[q][j];
                 fq[q] = f;
-                sq[q] = sweep_limit<T, LIMK>(lk, (f - shr1(f)) * rL, (shl1(f) - f) * rR);
+                sq[q] = sweep_limit<T, LIMK, HALF>(lk, (f - shr1(f)) * rL, (shl1(f) - f) * rR);
             }
             // upwind state and slope of the left face (flux_kfvs.jl:238-264, :303-340)
 #pragma unroll
             for (int q = 0; q < NF; q++) {
                 if constexpr (UPM == 1) {
-                    fu[q] = shr1(fq[q] + sq[q] * dxh);
+                    fu[q] = shr1(fq[q] + sq[q] * dxs);
                     su[q] = shr1(sq[q]);
                 } else if constexpr (UPM == 0) {
-                    fu[q] = fq[q] - sq[q] * dxh;
+                    fu[q] = fq[q] - sq[q] * dxs;
                     su[q] = sq[q];
                 } else {
-                    const T l = shr1(fq[q] + sq[q] * dxh), r = fq[q] - sq[q] * dxh;
+                    const T l = shr1(fq[q] + sq[q] * dxs), r = fq[q] - sq[q] * dxs;
                     const T sl = shr1(sq[q]);
                     fu[q] = up ? l : r;
                     su[q] = up ? sl : sq[q];
@@ -286,7 +295,7 @@ KBG_D void sweep1d(const Solve1D& S, const WorkItem& wi, int par, int lane, int 
             }
             if constexpr (FLUX == KFVS) {
 #pragma unroll
-                for (int q = 0; q < NF; q++) ffq[q] = fu[q] * ch.cu[j] + su[q] * ch.cs[j];
+                for (int q = 0; q < NF; q++) ffq[q] = fu[q] * ch.cu[j] + su[q] * ch.cs2[j];
             } else if constexpr (FLUX == KCU) {
                 // flux_kcu.jl:241-303 (1F1V), :463-530 (2F1V)
                 const T cI = u - F.Ui;
@@ -380,117 +389,173 @@ KBG_D void sweep1d(const Solve1D& S, const WorkItem& wi, int par, int lane, int 
         }
     });
 
-    // ---- add the wavefronts' sums (fixed order)
-    double tot[NR];
+    // ---- wavefront 0 adds the wavefronts' sums (fixed order) and runs the cells' step! chain;
+    // the others take its coefficients from LDS
+    T An, nl2, Un, inv, rTi, kbn, pre = 0, Ao = 0, nlo2 = 0, Uo = 0, lo2 = 0, kbo = 0;
+    T* coef = (T*)(lds + (size_t)NW * NR * kSweepLanes);  // [12][64]
     if (NW > 1) {
 #pragma unroll
         for (int m = 0; m < NR; m++) lds[(wave * NR + m) * kSweepLanes + lane] = acc[m];
         __syncthreads();
+    }
+    if (wave == 0) {
+        double tot[NR];
+        if (NW > 1) {
 #pragma unroll
-        for (int m = 0; m < NR; m++) {
-            double s = 0.0;
-            for (int w = 0; w < NW; w++) s += lds[(w * NR + m) * kSweepLanes + lane];
-            tot[m] = s;
+            for (int m = 0; m < NR; m++) {
+                double sm = 0.0;
+                for (int w = 0; w < NW; w++) sm += lds[(w * NR + m) * kSweepLanes + lane];
+                tot[m] = sm;
+            }
+        } else {
+#pragma unroll
+            for (int m = 0; m < NR; m++) tot[m] = acc[m];
         }
-    } else {
+        double wo[3], po[3] = {1.0, 0.0, 1.0};
 #pragma unroll
-        for (int m = 0; m < NR; m++) tot[m] = acc[m];
-    }
+        for (int m = 0; m < 3; m++) wo[m] = win[(size_t)kc * 3 + m];
+        if (SHAK) {
+#pragma unroll
+            for (int m = 0; m < 3; m++) po[m] = S.prim[(size_t)kc * 3 + m];
+        }
 
-    double wo[3], po[3] = {1.0, 0.0, 1.0};
+        // face moments of the lane's left face, and of its right face (the next lane's left)
+        double fwl[3];
+        if constexpr (FLUX == KFVS) {
+            fwl[0] = tot[0];
+            fwl[1] = tot[1];
+            fwl[2] = NF == 2 ? 0.5 * (tot[2] + tot[3]) : 0.5 * tot[2];
+        } else if constexpr (FLUX == KCU) {
+            fwl[0] = fw0[0] + m3 * tot[0];
+            fwl[1] = fw0[1] + m3 * tot[1];
+            fwl[2] = fw0[2] + m3 * 0.5 * (NF == 2 ? tot[2] + tot[3] : tot[2]);
+        } else {
+            fwl[0] = fw0[0] + (m3 * tot[0] - m4 * tot[1]);
+            fwl[1] = fw0[1] + (m3 * tot[2] - m4 * tot[3]);
+            fwl[2] = fw0[2] + (m3 * 0.5 * tot[4] - m4 * 0.5 * tot[5]);
+        }
+        if (bufL) {
 #pragma unroll
-    for (int m = 0; m < 3; m++) wo[m] = win[(size_t)kc * 3 + m];
-    if (SHAK) {
+            for (int m = 0; m < 3; m++) fwl[m] = S.bfw[bside * 3 + m];
+        }
+        double wn[3], pn[3];
 #pragma unroll
-        for (int m = 0; m < 3; m++) po[m] = S.prim[(size_t)kc * 3 + m];
-    }
+        for (int m = 0; m < 3; m++) wn[m] = wo[m] + (fwl[m] - shl1(fwl[m])) * rdx64;
 
-    // ---- face moments of the lane's left face, and of its right face (the next lane's left)
-    double fwl[3];
-    if constexpr (FLUX == KFVS) {
-        fwl[0] = tot[0];
-        fwl[1] = tot[1];
-        fwl[2] = NF == 2 ? 0.5 * (tot[2] + tot[3]) : 0.5 * tot[2];
-    } else if constexpr (FLUX == KCU) {
-        fwl[0] = fw0[0] + m3 * tot[0];
-        fwl[1] = fw0[1] + m3 * tot[1];
-        fwl[2] = fw0[2] + m3 * 0.5 * (NF == 2 ? tot[2] + tot[3] : tot[2]);
-    } else {
-        fwl[0] = fw0[0] + (m3 * tot[0] - m4 * tot[1]);
-        fwl[1] = fw0[1] + (m3 * tot[2] - m4 * tot[3]);
-        fwl[2] = fw0[2] + (m3 * 0.5 * tot[4] - m4 * 0.5 * tot[5]);
-    }
-    if (bufL) {
+        // the cell's step! coefficients (FP64 chain, lane-parallel)
+        conserve_prim<3>(wn, S.gam, pn);
+        const T rT = relax_ratio<T>(pn[0], pn[2], S.mu, S.om, dt);
+        inv = T(1) / (T(1) + rT);
+        rTi = rT * inv;
+        An = T(pn[0] * sqrt(pn[2] / kPi));
+        nl2 = T(-pn[2] * kLog2e);
+        Un = T(pn[1]);
+        kbn = T(K / (2.0 * pn[2]));
+        if (SHAK && shak) {
+            const double q = 0.5 * tot[NRAW];
+            pre = T(0.8 * (1.0 - S.Pr) * (po[2] * po[2]) / po[0] * q);
+            Ao = T(po[0] * sqrt(po[2] / kPi));
+            nlo2 = T(-po[2] * kLog2e);
+            Uo = T(po[1]);
+            lo2 = T(2.0 * po[2]);
+            kbo = T(K / (2.0 * po[2]));
+        }
+        if (upd) {
+            const size_t r = (size_t)kc * 3;
+            const gptr<double> wout = to_global(S.w[in ^ 1]);
+            const gptr<double> gp = to_global(S.prim);
 #pragma unroll
-        for (int m = 0; m < 3; m++) fwl[m] = S.bfw[bside * 3 + m];
-    }
-    double wn[3], pn[3];
+            for (int m = 0; m < 3; m++) {
+                wout[r + m] = wn[m];
+                gp[r + m] = pn[m];
+            }
+        }
+        if (rrow) {
+            // residual sums (solver_update.jl:86-88, update_boundary.jl:41-44): interior / boundary cells
+            double r[12];
+            const bool bnd = i == 1 || i == nx;
 #pragma unroll
-    for (int m = 0; m < 3; m++) wn[m] = wo[m] + (fwl[m] - shl1(fwl[m])) * rdx64;
-
-    // ---- the cell's step! coefficients (FP64 chain, lane-parallel)
-    conserve_prim<3>(wn, S.gam, pn);
-    const T rT = relax_ratio<T>(pn[0], pn[2], S.mu, S.om, dt);
-    const T inv = T(1) / (T(1) + rT), rTi = rT * inv;
-    const T An = T(pn[0] * sqrt(pn[2] / kPi)), nl2 = T(-pn[2] * kLog2e), Un = T(pn[1]);
-    const T kbn = T(K / (2.0 * pn[2]));
-    T pre = 0, Ao = 0, nlo2 = 0, Uo = 0, lo2 = 0, kbo = 0;
-    if (SHAK && shak) {
-        const double q = 0.5 * tot[NRAW];
-        pre = T(0.8 * (1.0 - S.Pr) * (po[2] * po[2]) / po[0] * q);
-        Ao = T(po[0] * sqrt(po[2] / kPi));
-        nlo2 = T(-po[2] * kLog2e);
-        Uo = T(po[1]);
-        lo2 = T(2.0 * po[2]);
-        kbo = T(K / (2.0 * po[2]));
-    }
-    if (wave == 0 && upd) {
-        const size_t r = (size_t)kc * 3;
-        const gptr<double> wout = to_global(S.w[in ^ 1]);
-        const gptr<double> gp = to_global(S.prim);
+            for (int m = 0; m < 3; m++) {
+                const double d2 = upd ? (wn[m] - wo[m]) * (wn[m] - wo[m]) : 0.0, a = upd ? fabs(wn[m]) : 0.0;
+                r[m] = bnd ? 0.0 : d2;
+                r[3 + m] = bnd ? 0.0 : a;
+                r[6 + m] = bnd ? d2 : 0.0;
+                r[9 + m] = bnd ? a : 0.0;
+            }
+            wave_allreduce<12>(r);
+            double y = 0.0;
 #pragma unroll
-        for (int m = 0; m < 3; m++) {
-            wout[r + m] = wn[m];
-            gp[r + m] = pn[m];
+            for (int m = 0; m < 12; m++)
+                if (lane == m) y = r[m];
+            if (lane < 12) rrow[lane] = y;
+        }
+        if (NW > 1) {
+            const T cv[12] = {An, nl2, Un, inv, rTi, kbn, pre, Ao, nlo2, Uo, lo2, kbo};
+#pragma unroll
+            for (int m = 0; m < 12; m++) coef[m * kSweepLanes + lane] = cv[m];
         }
     }
-    if (rrow && wave == 0) {
-        // residual sums (solver_update.jl:86-88, update_boundary.jl:41-44): interior / boundary cells
-        double r[12];
-        const bool bnd = i == 1 || i == nx;
-#pragma unroll
-        for (int m = 0; m < 3; m++) {
-            const double d2 = upd ? (wn[m] - wo[m]) * (wn[m] - wo[m]) : 0.0, a = upd ? fabs(wn[m]) : 0.0;
-            r[m] = bnd ? 0.0 : d2;
-            r[3 + m] = bnd ? 0.0 : a;
-            r[6 + m] = bnd ? d2 : 0.0;
-            r[9 + m] = bnd ? a : 0.0;
+    if (NW > 1) {
+        __syncthreads();
+        if (wave != 0) {
+            An = coef[0 * kSweepLanes + lane];
+            nl2 = coef[1 * kSweepLanes + lane];
+            Un = coef[2 * kSweepLanes + lane];
+            inv = coef[3 * kSweepLanes + lane];
+            rTi = coef[4 * kSweepLanes + lane];
+            kbn = coef[5 * kSweepLanes + lane];
+            if (SHAK) {
+                pre = coef[6 * kSweepLanes + lane];
+                Ao = coef[7 * kSweepLanes + lane];
+                nlo2 = coef[8 * kSweepLanes + lane];
+                Uo = coef[9 * kSweepLanes + lane];
+                lo2 = coef[10 * kSweepLanes + lane];
+                kbo = coef[11 * kSweepLanes + lane];
+            }
         }
-        wave_allreduce<12>(r);
-        double y = 0.0;
-#pragma unroll
-        for (int m = 0; m < 12; m++)
-            if (lane == m) y = r[m];
-        if (lane < 12) rrow[lane] = y;
     }
 
     // ---- pass 2: relaxation toward M (+ Shakhov) and the store of f^{n+1}
+    // M at the chunk's nodes: one exp per node, or -- FP32 on equally spaced nodes -- the product
+    // recurrence M(u + du) = M(u) R, R <- R Q with R = exp(-lam du (2 (u - U) + du)), Q = exp(-2 lam du^2),
+    // restarted every chunk of 8 nodes (at most ~30 roundings against the exp per node)
+    auto maxw = [&](VC& ch, T A, T nlam2, T U0, T (&M)[CH]) {
+        if (sizeof(T) == 4 && ch.du != T(0)) {
+            const T d = ch.du, c0 = ch.u[0] - U0;
+            T m = A * texp2(nlam2 * (c0 * c0));
+            T R = texp2(nlam2 * d * (T(2) * c0 + d));
+            const T Q = texp2(T(2) * nlam2 * (d * d));
+            static_for<CH>([&](auto jc) {
+                constexpr int j = jc;
+                M[j] = m;
+                if constexpr (j + 1 < CH) {
+                    m *= R;
+                    R *= Q;
+                }
+            });
+        } else {
+            static_for<CH>([&](auto jc) {
+                constexpr int j = jc;
+                const T c = ch.u[j] - U0;
+                M[j] = A * texp2(nlam2 * (c * c));
+            });
+        }
+    };
     static_for<VW / CH>([&](auto cc) {
         constexpr int c = cc;
         const int vb = vbeg + c * CH;
         if (vb >= vend) return;
         VC& ch = vt[vb / CH];
-        T fn[NF][CH];
+        T fn[NF][CH], Mn[CH], Mo[CH];
+        maxw(ch, An, nl2, Un, Mn);
+        if constexpr (SHAK) maxw(ch, Ao, nlo2, Uo, Mo);
         static_for<CH>([&](auto jc) {
             constexpr int j = jc;
-            const T u = ch.u[j];
-            const T cn = u - Un;
-            T MH = An * texp2(nl2 * (cn * cn));
+            T MH = Mn[j];
             T MB = MH * kbn;
             if constexpr (SHAK) {
-                const T co = u - Uo, c2 = co * co;
-                const T Mo = Ao * texp2(nlo2 * c2);
-                const T t = pre * co * Mo;
+                const T co = ch.u[j] - Uo, c2 = co * co;
+                const T t = pre * co * Mo[j];
                 if constexpr (NF == 1) {
                     MH += t * (lo2 * c2 - T(5));
                 } else {
@@ -527,37 +592,58 @@ KBG_D void sweep_coll(const Solve1D& S, const WorkItem& wi, int par, int lane, i
     else sweep1d<T, NF, FLUX, LIMK, false>(S, wi, par, lane, wave, NW, lds, rrow);
 }
 
-// one workgroup per work item; items of one XCD (workgroup ids congruent mod 8) are a contiguous
-// run of items, so neighbouring segments share that XCD's L2
+// LDS of one item (doubles): the wavefronts' moment sums and wavefront 0's cell coefficients
+template <class T, int NF, int FLUX> constexpr size_t sweep_lds_item(int nw) {
+    return nw > 1 ? (size_t)nw * SweepCfg<T, NF, FLUX>::NR * kSweepLanes + (12 * kSweepLanes * sizeof(T) + 7) / 8 : 0;
+}
+
 // occupancy target: 4 wavefronts per SIMD (<= 128 VGPRs) for the FP32 one-distribution sweeps,
 // whatever the register file allows for the others
 template <class T, int NF, int FLUX> constexpr int sweep_waves_per_eu() { return sizeof(T) == 4 && NF == 1 ? 4 : 1; }
 
+// One workgroup per pack; packs of one XCD (workgroup ids congruent mod 8) are a contiguous run,
+// so neighbouring segments share that XCD's L2.  A pack's items sit side by side: wavefront w
+// serves item w / nw as its wavefront w % nw.
 template <class T, int NF, int FLUX>
-__global__ __launch_bounds__(64 * kSweepMaxWaves) __attribute__((amdgpu_waves_per_eu(sweep_waves_per_eu<T, NF, FLUX>()))) void k_sweep1d(const Solve1D* __restrict__ solves, const WorkItem* __restrict__ items,
-                                                  int nitems, int par, double* __restrict__ rpart) {
+__global__ __launch_bounds__(64 * kSweepMaxWaves) __attribute__((amdgpu_waves_per_eu(sweep_waves_per_eu<T, NF, FLUX>()))) void k_sweep1d(
+    const Solve1D* __restrict__ solves, const WorkItem* __restrict__ items, const Pack* __restrict__ packs, int npacks, int par,
+    double* __restrict__ rpart) {
     extern __shared__ double kbg_sweep_lds[];
     const int G = gridDim.x, b = blockIdx.x;
     const int x = b & 7, q8 = G >> 3, r8 = G & 7;
-    const int it = x * q8 + (x < r8 ? x : r8) + (b >> 3);
-    if (it >= nitems) return;
+    const int pk = x * q8 + (x < r8 ? x : r8) + (b >> 3);
+    if (pk >= npacks) return;
+    const Pack P = packs[pk];
     const int lane = threadIdx.x & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int NW = blockDim.x >> 6;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int NW = P.nw;
+    const int sub = wv / NW, wave = wv - sub * NW;
+    if (sub >= P.count) {
+        // an unused slot still takes part in the item barriers (two when nw > 1)
+        if (NW > 1) {
+            __syncthreads();
+            __syncthreads();
+        }
+        return;
+    }
+    const int it = P.item0 + sub;
     const WorkItem wi = items[it];
     const Solve1D& S = solves[wi.solve];
+    double* lds = kbg_sweep_lds + (size_t)sub * sweep_lds_item<T, NF, FLUX>(NW);
     double* rrow = rpart ? rpart + (size_t)it * 12 : nullptr;
-    if (!S.interp2) sweep_coll<T, NF, FLUX, LIM_NONE>(S, wi, par, lane, wave, NW, kbg_sweep_lds, rrow);
-    else if (S.limiter == VANLEER) sweep_coll<T, NF, FLUX, LIM_VANLEER>(S, wi, par, lane, wave, NW, kbg_sweep_lds, rrow);
-    else sweep_coll<T, NF, FLUX, LIM_GENERIC>(S, wi, par, lane, wave, NW, kbg_sweep_lds, rrow);
+    if (!S.interp2) sweep_coll<T, NF, FLUX, LIM_NONE>(S, wi, par, lane, wave, NW, lds, rrow);
+    else if (S.limiter == VANLEER) sweep_coll<T, NF, FLUX, LIM_VANLEER>(S, wi, par, lane, wave, NW, lds, rrow);
+    else sweep_coll<T, NF, FLUX, LIM_GENERIC>(S, wi, par, lane, wave, NW, lds, rrow);
 }
 
 template <class T, int NF, int FLUX>
-hipError_t launch_sweep_t(int nw, const Solve1D* d_solves, const WorkItem* d_items, int nitems, int par, double* d_rpart,
-                          hipStream_t st) {
-    constexpr int NR = SweepCfg<T, NF, FLUX>::NR;
-    const size_t lds = nw > 1 ? (size_t)nw * NR * kSweepLanes * sizeof(double) : 0;
-    hipLaunchKernelGGL((k_sweep1d<T, NF, FLUX>), dim3(nitems), dim3(64 * nw), lds, st, d_solves, d_items, nitems, par, d_rpart);
+hipError_t launch_sweep_t(int wg_waves, const Solve1D* d_solves, const WorkItem* d_items, const Pack* d_packs, int npacks,
+                          int par, double* d_rpart, hipStream_t st) {
+    // the largest LDS a pack of this launch can need: wg_waves / nw items of nw wavefronts each
+    size_t lds = 0;
+    for (int nw = 2; nw <= wg_waves; nw *= 2) lds = std::max(lds, (size_t)(wg_waves / nw) * sweep_lds_item<T, NF, FLUX>(nw));
+    hipLaunchKernelGGL((k_sweep1d<T, NF, FLUX>), dim3(npacks), dim3(64 * wg_waves), lds * sizeof(double), st, d_solves, d_items,
+                       d_packs, npacks, par, d_rpart);
     return hipGetLastError();
 }
 

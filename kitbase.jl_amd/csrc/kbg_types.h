@@ -74,15 +74,20 @@ struct Solve2D {
 };
 
 // One unit of work of the fused 1D sweep: cells [c0, c1] (physical, at most 60) of solve `solve`,
-// updated by one workgroup of nw wavefronts that split the velocity grid.
+// updated by nw wavefronts (1, 2, 4 or 8) that split the velocity grid evenly in multiples of 8.
 struct WorkItem {
     int solve, c0, c1, nw;
+};
+// A workgroup's share of a sweep launch: items item0 .. item0 + count - 1, all with the same nw,
+// packed side by side (count nw <= the launch's wavefronts per workgroup).
+struct Pack {
+    int item0, count, nw, pad_;
 };
 
 // ---- launchers (kbg_sweep.h, kbg_1d.hip, kbg_2d.hip) --------------------------------------------------------
 // All return hipSuccess or the launch error; `fp64` selects the storage type.
-hipError_t launch_sweep1d(bool fp64, int nf, int flux, int nw, const Solve1D* d_solves, const WorkItem* d_items,
-                          int nitems, int par, double* d_rpart, hipStream_t st);
+hipError_t launch_sweep1d(bool fp64, int nf, int flux, int wg_waves, const Solve1D* d_solves, const WorkItem* d_items,
+                          const Pack* d_packs, int npacks, int par, double* d_rpart, hipStream_t st);
 // velocities per wavefront of the sweep for a storage type / distribution count (a multiple of 8),
 // and the largest velocity grid it takes (8 wavefronts)
 #ifndef KBG_SWEEP_VW32
@@ -92,6 +97,14 @@ constexpr int kSweepMaxWaves = 8;
 constexpr int sweep_vw_raw(bool fp64, int nf) { return (fp64 ? KBG_SWEEP_VW32 / 2 : KBG_SWEEP_VW32) / nf; }
 constexpr int sweep_vw(bool fp64, int nf) { return sweep_vw_raw(fp64, nf) < 8 ? 8 : sweep_vw_raw(fp64, nf); }
 constexpr int sweep_max_nv(bool fp64, int nf) { return kSweepMaxWaves * sweep_vw(fp64, nf); }
+// wavefronts of a sweep item: the smallest power of two whose wavefronts hold nv velocities
+constexpr int sweep_nw(bool fp64, int nf, int nv) {
+    return (nv + sweep_vw(fp64, nf) - 1) / sweep_vw(fp64, nf) <= 1 ? 1
+         : (nv + sweep_vw(fp64, nf) - 1) / sweep_vw(fp64, nf) <= 2 ? 2
+         : (nv + sweep_vw(fp64, nf) - 1) / sweep_vw(fp64, nf) <= 4 ? 4 : 8;
+}
+// velocities per wavefront of an item (a multiple of 8, at most sweep_vw)
+constexpr int sweep_item_vw(int nv, int nw) { return ((nv + nw - 1) / nw + 7) / 8 * 8; }
 hipError_t launch_recon1d(bool fp64, const Solve1D& S, const Solve1D* d_S, hipStream_t st);
 hipError_t launch_evolve1d(bool fp64, const Solve1D& S, const Solve1D* d_S, hipStream_t st);
 hipError_t launch_wall1d(bool fp64, const Solve1D& S, const Solve1D* d_S, int fused, int par, hipStream_t st);
