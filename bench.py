@@ -93,15 +93,23 @@ def main():
                     help="only copy MIB MiB device-to-device 5 times (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     a = ap.parse_args()
     if a.copy_probe:
+        # device-to-device hipMemcpy (the runtime's blit kernel) of a known size: reads and writes
+        # go through L2 like the sweep's, which calibrates the FETCH_SIZE / WRITE_SIZE counters
+        import ctypes
+
         import torch
 
-        n = a.copy_probe * (1 << 20) // 4
-        x = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        n = a.copy_probe * (1 << 20)
+        x = torch.ones(n // 4, dtype=torch.float32, device="cuda:0")
         y = torch.empty_like(x)
+        hip = ctypes.CDLL("libamdhip64.so")
+        hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
         for _ in range(5):
-            y.copy_(x)
+            rc = hip.hipMemcpy(ctypes.c_void_p(y.data_ptr()), ctypes.c_void_p(x.data_ptr()), n, 3)  # 3: device to device
+            if rc != 0:
+                raise RuntimeError(f"hipMemcpy failed: {rc}")
         torch.cuda.synchronize()
-        print(json.dumps({"copy_probe_bytes_read": 4 * n, "copy_probe_bytes_written": 4 * n, "copies": 5}))
+        print(json.dumps({"copy_probe_bytes_read": n, "copy_probe_bytes_written": n, "copies": 5}))
         return
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only: the batched solves are independent, so N GPUs would each "
