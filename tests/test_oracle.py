@@ -603,3 +603,146 @@ def test_kcu_analytic_face_moments_match_its_distribution_flux(mu):
          1.0, 2e-3)
     mom = np.array([np.dot(w, fh), np.dot(w, u * fh), 0.5 * (np.dot(w, u * u * fh) + np.dot(w, fb))])
     assert np.allclose(mom, fw, rtol=1e-8, atol=1e-12)
+
+
+# ---------------------------------------------------------------- 2V / 3V kernels
+
+def tensor_grid(n, a, b, dims):
+    u1, w1 = rect_grid(n, a, b)
+    mesh = np.meshgrid(*([u1] * dims), indexing="ij")
+    wts = np.prod(np.meshgrid(*([w1] * dims), indexing="ij"), axis=0)
+    # first velocity index fastest (column-major flattening, as the oracle stores 2V / 3V grids)
+    return [A(m.ravel(order="F")) for m in mesh] + [A(wts.ravel(order="F"))]
+
+
+def euler_flux_2d(prim, K):
+    rho, U, V, lam = prim
+    p = 0.5 * rho / lam
+    E = 0.5 * rho * (U * U + V * V) + 0.5 * (K + 2) * p
+    return np.array([rho * U, rho * U * U + p, rho * U * V, U * (E + p)])
+
+
+def _two_f_2v(prim, u, v, K):
+    rho, U, V, lam = prim
+    H = rho * (lam / np.pi) * np.exp(-lam * ((u - U) ** 2 + (v - V) ** 2))
+    return A(H), A(H * K / (2 * lam))
+
+
+def test_ref_vhs_vis_hard_sphere_value():
+    """Hard spheres (alpha = 1, omega = 0.5): mu_ref = 5 sqrt(pi) / 16 Kn (Bird's VHS reference viscosity)."""
+    assert call("orc_ref_vhs_vis", 1e-4, 1.0, 0.5) == pytest.approx(5 * np.sqrt(np.pi) / 16 * 1e-4, rel=1e-14)
+    # tau = mu(T) / p with the VHS power law mu(T) = mu_ref (T / T_ref)^omega, T / T_ref = 1 / lambda
+    # (reference state lambda = 1) and p = rho / (2 lambda)
+    rho, mu = 1.7, 3e-3
+    for om in (0.5, 0.72, 0.81):
+        for lam in (0.3, 1.0, 2.5):
+            tau = call("orc_vhs_collision_time", rho, lam, mu, om)
+            assert tau == pytest.approx(mu * lam ** (-om) / (rho / (2 * lam)), rel=1e-14)
+
+
+@pytest.mark.parametrize("flux", ["kfvs", "kcu", "ugks"])
+def test_2f2v_uniform_maxwellian_gives_euler_flux(flux):
+    u, v, w = tensor_grid(96, -9.0, 9.0, 2)
+    K = 1.0
+    gam = (K + 4) / (K + 2)
+    prim = [1.1, 0.3, -0.2, 0.9]
+    h, b = _two_f_2v(prim, u, v, K)
+    n = u.size
+    W = A(np.zeros(4)); call("orc_prim_conserve", A(prim), 4, gam, W)
+    fw = A(np.zeros(4)); fh = A(np.zeros(n)); fb = A(np.zeros(n)); z = A(np.zeros(n))
+    dt, length = 2e-3, 0.7
+    if flux == "kfvs":
+        call("orc_flux_kfvs_2f2v", fw, fh, fb, h, b, h, b, u, v, w, n, dt, length, z, z, z, z)
+    elif flux == "kcu":
+        call("orc_flux_kcu_2f2v", fw, fh, fb, W, h, b, W, h, b, u, v, w, n, K, gam, 1e-2, 0.72, 1.0, dt, length)
+    else:
+        call("orc_flux_ugks_2f2v", fw, fh, fb, W, h, b, W, h, b, u, v, w, n, K, gam, 1e-2, 0.72, 1.0, dt, 0.05, 0.05, length,
+             z, z, z, z)
+    assert np.allclose(fw, dt * length * euler_flux_2d(prim, K), rtol=1e-8, atol=1e-13)
+
+
+@pytest.mark.parametrize("flux", ["kfvs", "kcu"])
+def test_1f3v_uniform_maxwellian_gives_euler_flux(flux):
+    u, v, wv, w = tensor_grid(40, -8.0, 8.0, 3)
+    gam = 5.0 / 3.0
+    prim = [0.8, 0.4, 0.1, -0.3, 0.7]
+    rho, U, V, Wv, lam = prim
+    f = A(rho * (lam / np.pi) ** 1.5 * np.exp(-lam * ((u - U) ** 2 + (v - V) ** 2 + (wv - Wv) ** 2)))
+    n = u.size
+    fw = A(np.zeros(5)); ff = A(np.zeros(n)); z = A(np.zeros(n))
+    Wc = A(np.zeros(5)); call("orc_prim_conserve", A(prim), 5, gam, Wc)
+    dt, length = 1e-3, 1.0
+    if flux == "kfvs":
+        call("orc_flux_kfvs_1f3v", fw, ff, f, f, u, v, wv, w, n, dt, length, z, z)
+    else:
+        call("orc_flux_kcu_1f3v", fw, ff, Wc, f, Wc, f, u, v, wv, w, n, 0.0, gam, 1e-2, 0.72, 1.0, dt, length)
+    p = 0.5 * rho / lam
+    E = 0.5 * rho * (U * U + V * V + Wv * Wv) + 1.5 * p
+    ref = np.array([rho * U, rho * U * U + p, rho * U * V, rho * U * Wv, U * (E + p)])
+    assert np.allclose(fw, dt * ref, rtol=1e-7, atol=1e-13)
+
+
+@pytest.mark.parametrize("rot", [1, -1])
+def test_maxwell_wall_2f2v_zero_mass_flux_and_pressure(rot):
+    u, v, w = tensor_grid(48, -8.0, 8.0, 2)
+    K = 1.0
+    n = u.size
+    bc = A([1.0, 0.0, 0.1, 0.9])  # wall moving tangentially
+    rng = np.random.default_rng(17)
+    h = A(rng.uniform(0.0, 1.0, n) * np.exp(-0.5 * (u * u + v * v))); b = A(0.5 * h)
+    fw = A(np.zeros(4)); fh = A(np.zeros(n)); fb = A(np.zeros(n))
+    call("orc_flux_boundary_maxwell_2f2v", fw, fh, fb, bc, h, b, u, v, w, n, K, 1e-3, 0.5, rot)
+    assert abs(fw[0]) < 1e-16
+    prim = [0.9, 0.0, 0.0, 0.9]
+    h, b = _two_f_2v(prim, u, v, K)
+    call("orc_flux_boundary_maxwell_2f2v", fw, fh, fb, A([1.0, 0.0, 0.0, 0.9]), h, b, u, v, w, n, K, 1e-3, 0.5, rot)
+    assert np.allclose(fw, [0.0, 1e-3 * 0.5 * 0.5 * 0.9 / 0.9, 0.0, 0.0], atol=1e-13)
+
+
+def test_step_2f2v_bgk_fixed_point():
+    u, v, w = tensor_grid(32, -7.0, 7.0, 2)
+    K = 1.0
+    gam = (K + 4) / (K + 2)
+    prim = A([1.2, 0.2, -0.1, 0.8])
+    h, b = _two_f_2v(prim, u, v, K)
+    h0, b0 = h.copy(), b.copy()
+    W = A(np.zeros(4)); call("orc_prim_conserve", prim, 4, gam, W)
+    z4, zn = A(np.zeros(4)), A(np.zeros(u.size))
+    RES, AVG = A(np.zeros(4)), A(np.zeros(4))
+    call("orc_step_2f2v", W, prim, h, b, z4, zn, zn, z4, zn, zn, z4, zn, zn, z4, zn, zn, u, v, w, u.size, K, gam, 1e-3, 0.72,
+         1.0, 0.01, 1e-3, RES, AVG, 0)
+    assert np.allclose(h, h0, rtol=1e-12) and np.allclose(b, b0, rtol=1e-12)
+
+
+def test_pdf_slope_2d_inverts_the_moment_map():
+    """4-moment pdf_slope (2V + K internal dofs, K = 1) against Gauss-Hermite quadrature."""
+    K = 1.0
+    xg, wg = np.polynomial.hermite.hermgauss(20)
+    rng = np.random.default_rng(23)
+    prim = A([1.3, 0.2, -0.4, 0.7])
+    sw = A(rng.uniform(-1, 1, 4))
+    a = A(np.zeros(4))
+    call("orc_pdf_slope", prim, sw, 4, K, a)
+    s = 1 / np.sqrt(prim[3])  # exp(-lambda c^2) = exp(-(c / s)^2)
+    U, V = prim[1], prim[2]
+    uu, vv, zz = np.meshgrid(U + s * xg, V + s * xg, s * xg, indexing="ij")
+    ww = np.einsum("i,j,k->ijk", wg, wg, wg) / np.pi ** 1.5
+    x2 = zz * zz  # one internal dof
+    poly = a[0] + a[1] * uu + a[2] * vv + 0.5 * a[3] * (uu * uu + vv * vv + x2)
+    psi = [np.ones_like(uu), uu, vv, 0.5 * (uu * uu + vv * vv + x2)]
+    got = [prim[0] * np.sum(ww * p * poly) for p in psi]
+    assert np.allclose(got, sw, rtol=1e-10, atol=1e-12)
+
+
+def test_pspace2d_uniform_geometry():
+    nx, ny = 4, 3
+    x = A(np.zeros(nx * ny)); y = A(np.zeros(nx * ny)); dx = A(np.zeros(nx * ny)); dy = A(np.zeros(nx * ny))
+    areas = A(np.zeros(nx * ny * 4)); normals = A(np.zeros(nx * ny * 8))
+    call("orc_pspace2d", 0.0, 2.0, nx, 0.0, 1.5, ny, x, y, dx, dy, areas, normals)
+    assert np.allclose(dx, 0.5) and np.allclose(dy, 0.5)
+    assert np.allclose(np.sort(np.unique(np.round(x, 12))), [0.25, 0.75, 1.25, 1.75])
+    # edges 1..4 = y-, x+, y+, x- : lengths dx, dy, dx, dy and outward unit normals
+    a4 = areas.reshape(-1, 4)
+    assert np.allclose(a4, [[0.5, 0.5, 0.5, 0.5]])
+    n4 = normals.reshape(-1, 4, 2)
+    assert np.allclose(np.abs(n4).sum(axis=2), 1.0, atol=1e-5)
