@@ -89,6 +89,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 single-solve comparison")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU work of the baseline sample")
+    ap.add_argument("--serve", type=int, default=48, metavar="N",
+                    help="serving leg: N requests of the mixed stream (0 skips it)")
+    ap.add_argument("--serve-rate", type=float, default=100.0, help="Poisson arrival rate of the serving leg (1/s)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 5 times (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     a = ap.parse_args()
@@ -199,6 +202,19 @@ def main():
         log(f"[bench] worst instance {worst_case[0]}: errors {worst_case[1]}, nx {worst_case[2]}, nv {worst_case[3]}, "
             f"{worst_case[4]}, Kn {worst_case[5]:.3e}, max|U| {worst_case[6]:.3e}, max|dU| {worst_case[7]:.3e}")
 
+    # ---- serving leg: Poisson arrivals of 1D tubes, 2D cavities and 1D1F3V requests, served together
+    serve = None
+    if a.serve:
+        reqs = [kb.serving.Request(rid=k, cfg=cfg, nsteps=ns, arrival=t)
+                for k, (kind, cfg, ns, t) in enumerate(kb.configs.serving_requests(a.serve, a.serve_rate))]
+        srv = kb.serving.Server(kb.serving.DeviceBackend(kb, ctx=ctx), budget_bytes=32 << 30, chunk_steps=25)
+        st = srv.run(reqs)
+        serve = dict(st.summary(), rate_per_s=a.serve_rate, budget_bytes=32 << 30, chunk_steps=25,
+                     mix="75% 1D tubes (to t=0.2), 15% 2D cavities 32^2-48^2 x 16^2-24^2 (100 steps), "
+                         "10% 1D1F3V 24 cells x 24^3 (40 steps); FP32")
+        log(f"[bench] serving: {serve['requests']} requests at {a.serve_rate}/s, p50 {serve['p50_ms']:.1f} ms, "
+            f"p99 {serve['p99_ms']:.1f} ms, {serve['updates_per_s']:.3e} updates/s, max live {serve['max_live']}")
+
     # ---- CPU baseline: FP64 oracle with OpenMP on a bounded sample
     cpu = None
     if not a.no_cpu:
@@ -249,7 +265,8 @@ def main():
         "fp64_plain": fp64,
         "speedup_vs_fp64_plain": (value / fp64["value"]) if fp64 else None,
         "l2_vs_fp64": l2,
-        "p99_solve_latency_ms": None,
+        "p99_solve_latency_ms": serve["p99_ms"] if serve else None,
+        "serving": serve,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_sweep1d<float,1F,KFVS>: one launch per wavefront count (1..8) per step",

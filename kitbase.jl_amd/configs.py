@@ -91,3 +91,34 @@ def riemann_1d1f1v(inst: dict) -> dict:
                 omegaRef=0.5,
                 primL=[inst["rhoL"], 0.0, 1.0 / (2.0 * inst["TL"])],
                 primR=[inst["rhoR"], 0.0, 1.0 / (2.0 * inst["TR"])])
+
+
+SERVE_SEED = 20260318
+
+
+def serving_requests(n: int = 48, rate: float = 100.0, seed: int = SERVE_SEED, mix=(0.75, 0.15, 0.10),
+                     cavity_cells=(32, 48), cavity_vel=(16, 24), v3_cells: int = 24, cavity_steps: int = 100,
+                     v3_steps: int = 40):
+    """The serving stream: Poisson arrivals at `rate` requests/s mixing
+      * 1D shock tubes of the batched workload (run to t_end = 0.2, KitBase's nt = floor(T/dt) + 1),
+      * lid-driven cavities (2D2F2V UGKS) at cavity_cells^2 cells x cavity_vel^2 velocities
+        (Maxwell quadrature up to 32 per axis, a rectangle grid above),
+      * 1D1F3V shock structures (24^3 velocities, Rankine-Hugoniot states, BGK).
+    Returns (kind, cfg, nsteps or None, arrival) tuples; nsteps None means "to maxTime"."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    inst = batch_instances(n, seed)
+    for k in range(n):
+        t += float(rng.exponential(1.0 / rate))
+        kind = rng.choice(3, p=np.asarray(mix) / np.sum(mix))
+        if kind == 0:
+            out.append(("1d", riemann_1d1f1v(inst[k]), None, t))
+        elif kind == 1:
+            nc = int(rng.choice(cavity_cells))
+            nvel = int(rng.choice(cavity_vel))
+            out.append(("2d", cavity_2d2f2v(nx=nc, ny=nc, nu=nvel, nv=nvel,
+                                            vMeshType="maxwell" if nvel <= 32 else "rectangle"), cavity_steps, t))
+        else:
+            out.append(("3v", shock_1d1f3v(nx=v3_cells, nu=24, nv=24, nw=24), v3_steps, t))
+    return out

@@ -1,0 +1,168 @@
+"""Serving loop: many solve requests on one GPU, resident in HBM, stepped together.
+
+Requests arrive over time (Poisson arrivals in the benchmark).  The loop
+  * admits waiting requests first come, first served while their device footprint
+    (kbg_solve_bytes) fits the memory budget; an admitted request is uploaded once and stays
+    resident until it finishes;
+  * steps every admitted request together with kbg_batch_step, in chunks: a chunk is at most
+    `chunk_steps` steps and never more than the fewest steps any admitted request still needs, so
+    a finished request leaves and waiting requests join between chunks (continuous batching);
+  * downloads a finished request's state, frees it and records its time to solution (arrival to
+    completion).
+Each request's arithmetic is the same alone or in any mix (solves never share data; their sums have
+a fixed order), so results do not depend on what else is being served.
+
+The loop talks to the device through a backend object (DeviceBackend here; tests use a fake one).
+"""
+from __future__ import annotations
+
+import copy
+import time
+from dataclasses import dataclass, field
+from typing import Any
+
+import numpy as np
+
+
+@dataclass
+class Request:
+    rid: int
+    cfg: dict                 # KitBase config dict (SolverSet input)
+    nsteps: int | None        # time steps to run; None: to maxTime (KitBase's solve!: nt = floor(T / dt) + 1)
+    arrival: float            # seconds after the start of the run
+    precision: str = "fp32"
+    setup: Any = None         # optional callable(ks, ctr, face) that overrides the initial state
+    # filled by the server
+    admitted: float = -1.0
+    done: float = -1.0
+    result: Any = None        # host ControlVolumes after the last step
+    points: int = 0           # cell x velocity points of one step
+
+    @property
+    def latency(self) -> float:
+        return self.done - self.arrival
+
+
+class DeviceBackend:
+    """Solve lifecycle on libkbgpu.so through the host mirror (kitbase_amd)."""
+
+    def __init__(self, kb, ctx=None):
+        self.kb = kb
+        self.ctx = ctx or kb.Context(0)
+
+    def prepare(self, req: Request):
+        ks = self.kb.SolverSet(req.cfg)
+        ctr, face = self.kb.init_fvm(ks)
+        if req.setup is not None:
+            req.setup(ks, ctr, face)
+        dt = self.kb.timestep(ks, ctr)
+        if req.nsteps is None:
+            req.nsteps = int(np.floor(ks.set.maxTime / dt)) + 1
+        nx = ks.ps.nx * (ks.ps.ny if ks.dim == 2 else 1)
+        req.points = int(nx * ks.vs.nv)
+        return ks, ctr, face, dt
+
+    def footprint(self, prepared, req: Request) -> int:
+        return self.kb.solve_bytes(prepared[0], req.precision)
+
+    def create(self, prepared, req: Request):
+        ks, ctr, face, dt = prepared
+        return self.kb.DeviceSolve(ks, ctr, face, precision=req.precision, ctx=self.ctx), dt
+
+    def step(self, handles, dts, nsteps: int):
+        self.kb.batch_step(handles, dts, nsteps, ctx=self.ctx)
+
+    def finish(self, handle, prepared):
+        ctr = copy.deepcopy(prepared[1])
+        handle.download(ctr)
+        handle.close()
+        return ctr
+
+
+@dataclass
+class _Live:
+    req: Request
+    handle: Any
+    dt: float
+    left: int
+    nbytes: int
+    prepared: Any
+
+
+@dataclass
+class ServeStats:
+    latencies_ms: list = field(default_factory=list)
+    updates: float = 0.0
+    wall_s: float = 0.0
+    chunks: int = 0
+    max_live: int = 0
+    peak_bytes: int = 0
+
+    def summary(self) -> dict:
+        lat = np.asarray(self.latencies_ms)
+        return {
+            "requests": int(lat.size),
+            "p50_ms": float(np.percentile(lat, 50)) if lat.size else None,
+            "p99_ms": float(np.percentile(lat, 99)) if lat.size else None,
+            "max_ms": float(lat.max()) if lat.size else None,
+            "updates_per_s": self.updates / self.wall_s if self.wall_s > 0 else 0.0,
+            "chunks": self.chunks,
+            "max_live": self.max_live,
+            "peak_bytes": self.peak_bytes,
+        }
+
+
+class Server:
+    def __init__(self, backend, budget_bytes: int, chunk_steps: int = 25, clock=time.perf_counter, sleep=time.sleep):
+        if chunk_steps < 1:
+            raise ValueError("chunk_steps must be >= 1")
+        self.backend = backend
+        self.budget = int(budget_bytes)
+        self.chunk = int(chunk_steps)
+        self.clock = clock
+        self.sleep = sleep
+
+    def run(self, requests: list[Request]) -> ServeStats:
+        stats = ServeStats()
+        pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
+        waiting: list[tuple[Request, Any, int]] = []
+        live: list[_Live] = []
+        used = 0
+        t0 = self.clock()
+        now = lambda: self.clock() - t0
+        while pending or waiting or live:
+            t = now()
+            while pending and pending[0].arrival <= t:
+                r = pending.pop(0)
+                prep = self.backend.prepare(r)
+                nb = self.backend.footprint(prep, r)
+                if nb > self.budget:
+                    raise ValueError(f"request {r.rid} needs {nb} bytes, more than the budget {self.budget}")
+                waiting.append((r, prep, nb))
+            # first come, first served while the footprint fits
+            while waiting and used + waiting[0][2] <= self.budget:
+                r, prep, nb = waiting.pop(0)
+                h, dt = self.backend.create(prep, r)
+                r.admitted = now()
+                live.append(_Live(r, h, dt, r.nsteps, nb, prep))
+                used += nb
+            stats.max_live = max(stats.max_live, len(live))
+            stats.peak_bytes = max(stats.peak_bytes, used)
+            if not live:
+                if pending:
+                    self.sleep(max(0.0, pending[0].arrival - now()))
+                continue
+            k = min(self.chunk, min(l.left for l in live))
+            self.backend.step([l.handle for l in live], [l.dt for l in live], k)
+            stats.chunks += 1
+            for l in live:
+                l.left -= k
+                stats.updates += float(k) * l.req.points
+            for l in [l for l in live if l.left == 0]:
+                l.req.result = self.backend.finish(l.handle, l.prepared)
+                l.req.done = now()
+                stats.latencies_ms.append(1e3 * l.req.latency)
+                used -= l.nbytes
+                live.remove(l)
+        stats.wall_s = now()
+        return stats

@@ -1,0 +1,115 @@
+"""The serving loop's scheduling on the CPU, with a fake device backend: admission by footprint,
+first come first served, chunks that never overshoot a request, continuous batching, latency
+bookkeeping.  (The GPU version of the loop is exercised in test_gpu_serving.py.)"""
+import pytest
+
+from conftest import kb
+
+serving = kb.serving
+
+
+class FakeClock:
+    def __init__(self):
+        self.t = 0.0
+
+    def __call__(self):
+        return self.t
+
+    def sleep(self, dt):
+        self.t += max(dt, 0.0)
+
+
+class FakeBackend:
+    """Each step of a chunk costs step_cost seconds of fake time; footprints from the request."""
+
+    def __init__(self, clock, step_cost=1e-3):
+        self.clock = clock
+        self.step_cost = step_cost
+        self.live = set()
+        self.calls = []
+
+    def prepare(self, r):
+        r.points = 100
+        return r
+
+    def footprint(self, prep, r):
+        return r.cfg["bytes"]
+
+    def create(self, prep, r):
+        self.live.add(r.rid)
+        return r.rid, 1e-3
+
+    def step(self, handles, dts, nsteps):
+        assert set(handles) <= self.live
+        self.calls.append((tuple(handles), nsteps))
+        self.clock.t += self.step_cost * nsteps
+
+    def finish(self, h, prep):
+        self.live.remove(h)
+        return ("state", h)
+
+
+def make(reqs, budget, chunk=10):
+    clock = FakeClock()
+    be = FakeBackend(clock)
+    srv = serving.Server(be, budget, chunk_steps=chunk, clock=clock, sleep=clock.sleep)
+    return srv, be, clock
+
+
+def R(rid, nsteps, arrival, nbytes=10):
+    return serving.Request(rid=rid, cfg={"bytes": nbytes}, nsteps=nsteps, arrival=arrival)
+
+
+def test_chunks_never_overshoot_and_all_finish():
+    reqs = [R(0, 25, 0.0), R(1, 7, 0.0), R(2, 40, 0.005)]
+    srv, be, clock = make(reqs, budget=100, chunk=10)
+    st = srv.run(reqs)
+    done = {}
+    for handles, n in be.calls:
+        for h in handles:
+            done[h] = done.get(h, 0) + n
+    assert done == {0: 25, 1: 7, 2: 40}
+    assert all(r.result == ("state", r.rid) for r in reqs)
+    assert st.summary()["requests"] == 3 and not be.live
+
+
+def test_admission_by_memory_is_fifo():
+    # budget fits two requests; the third waits for the first to finish
+    reqs = [R(0, 10, 0.0, 40), R(1, 30, 0.0, 40), R(2, 10, 0.0, 40)]
+    srv, be, clock = make(reqs, budget=80, chunk=10)
+    st = srv.run(reqs)
+    assert st.max_live == 2 and st.peak_bytes == 80
+    assert reqs[2].admitted >= reqs[0].done  # admitted only after request 0 freed its memory
+    assert set(be.calls[0][0]) == {0, 1}
+
+
+def test_continuous_batching_joins_between_chunks():
+    # request 1 arrives while request 0 is being stepped and joins at the next chunk boundary
+    reqs = [R(0, 50, 0.0), R(1, 10, 0.012)]
+    srv, be, clock = make(reqs, budget=100, chunk=10)
+    srv.run(reqs)
+    assert be.calls[0][0] == (0,)
+    assert any(set(h) == {0, 1} for h, _ in be.calls)
+    assert reqs[1].done < reqs[0].done
+
+
+def test_latency_is_arrival_to_completion():
+    reqs = [R(0, 20, 0.5)]
+    srv, be, clock = make(reqs, budget=100, chunk=10)
+    st = srv.run(reqs)
+    assert reqs[0].latency == pytest.approx(20 * 1e-3)
+    assert st.summary()["p99_ms"] == pytest.approx(20.0)
+
+
+def test_request_larger_than_budget_is_refused():
+    reqs = [R(0, 5, 0.0, 1000)]
+    srv, be, clock = make(reqs, budget=100)
+    with pytest.raises(ValueError):
+        srv.run(reqs)
+
+
+def test_serving_mix_is_reproducible():
+    a = kb.configs.serving_requests(20, 50.0)
+    b = kb.configs.serving_requests(20, 50.0)
+    assert [(k, t) for k, _, _, t in a] == [(k, t) for k, _, _, t in b]
+    assert {k for k, *_ in kb.configs.serving_requests(200, 50.0)} == {"1d", "2d", "3v"}
