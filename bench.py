@@ -85,10 +85,12 @@ def main():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--instances", type=int, default=512)
-    ap.add_argument("--seg", type=int, default=0, help="cells per work item (0: library default)")
+    ap.add_argument("--seg", type=int, default=0, help="cells per work item (0: chosen by the autotuner)")
+    ap.add_argument("--no-tune", action="store_true", help="skip the schedule autotuner (library defaults)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 single-solve comparison")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU work of the baseline sample")
+    ap.add_argument("--cavity-steps", type=int, default=20, help="timed steps of the 2D cavity leg (0 skips it)")
     ap.add_argument("--serve", type=int, default=48, metavar="N",
                     help="serving leg: N requests of the mixed stream (0 skips it)")
     ap.add_argument("--serve-rate", type=float, default=100.0, help="Poisson arrival rate of the serving leg (1/s)")
@@ -132,16 +134,26 @@ def main():
         f"(host setup {time.time() - t0:.1f}s)")
 
     ctx = kb.Context(0)
-    if a.seg:
-        ctx.set_schedule(a.seg, 0)
     dts = np.array([d for *_, d in inst])
 
     # ---- FP32 batched path
     t0 = time.time()
     devs = [kb.DeviceSolve(ks, ctr, face, precision="fp32", ctx=ctx) for ks, ctr, face, _ in inst]
-    log(f"[bench] fp32 upload {time.time() - t0:.1f}s; warmup {W} steps")
+    log(f"[bench] fp32 upload {time.time() - t0:.1f}s")
+    tuned, taken = None, 0
+    if a.seg:
+        ctx.set_schedule(a.seg, 0)
+    elif not a.no_tune:
+        # the tuner's trial steps are warm-up steps: they count toward the steps compared with FP64
+        best, trials, taken = kb.autotune.tune(ctx, devs, dts)
+        tuned = {"best": {"seg_cells": best[0], "streams": best[1]},
+                 "ms_per_step": {f"seg{k[0]}_streams{k[1]}": round(v, 4) for k, v in trials.items()}}
+        log(f"[bench] autotuner: seg_cells {best[0]}, streams {best[1]} ({taken} trial steps): "
+            + ", ".join(f"{k}: {v:.3f} ms" for k, v in trials.items()))
     if W:
         kb.batch_step(devs, dts, W, ctx=ctx)
+    W += taken
+    log(f"[bench] warmup {W} steps in total")
     ctx.sync()
     t0 = time.perf_counter()
     kb.batch_step(devs, dts, K, ctx=ctx)
@@ -201,6 +213,24 @@ def main():
             f"max rel L2 rho {worst[0]:.2e} U {worst[1]:.2e} T {worst[2]:.2e}")
         log(f"[bench] worst instance {worst_case[0]}: errors {worst_case[1]}, nx {worst_case[2]}, nv {worst_case[3]}, "
             f"{worst_case[4]}, Kn {worst_case[5]:.3e}, max|U| {worst_case[6]:.3e}, max|dU| {worst_case[7]:.3e}")
+
+    # ---- 2D lid-driven cavity (64^2 cells x 32^2 velocities, UGKS, Maxwell walls), phase by phase
+    cavity = None
+    if a.cavity_steps:
+        cavity = {"config": "2D2F2V cavity 64^2 x 32^2, UGKS + BGK, Kn 0.075", "steps": a.cavity_steps}
+        for prec in ("fp32", "fp64"):
+            ks = kb.SolverSet(kb.configs.cavity_2d2f2v())
+            ctr, face = kb.init_fvm(ks)
+            dt = kb.timestep(ks, ctr)
+            d = kb.DeviceSolve(ks, ctr, face, precision=prec, ctx=ctx)
+            d.step(dt, 2)
+            t1 = time.perf_counter()
+            d.step(dt, a.cavity_steps)
+            wall_c = time.perf_counter() - t1
+            d.close()
+            pts = ks.ps.nx * ks.ps.ny * ks.vs.nv
+            cavity[prec] = {"updates_per_s": pts * a.cavity_steps / wall_c, "ms_per_step": wall_c * 1e3 / a.cavity_steps}
+        log(f"[bench] cavity: fp32 {cavity['fp32']['ms_per_step']:.3f} ms/step, fp64 {cavity['fp64']['ms_per_step']:.3f} ms/step")
 
     # ---- serving leg: Poisson arrivals of 1D tubes, 2D cavities and 1D1F3V requests, served together
     serve = None
@@ -262,6 +292,8 @@ def main():
                                "Kn 10^U(-4,1), BGK/Shakhov, 1D1F1V KFVS, FP32 f / FP64 w",
                    "instances": len(inst), "seed": kb.configs.BATCH_SEED, "cells": cells,
                    "points_per_step": points, "parallelism": "single GPU"},
+        "schedule": tuned,
+        "cavity_2d": cavity,
         "fp64_plain": fp64,
         "speedup_vs_fp64_plain": (value / fp64["value"]) if fp64 else None,
         "l2_vs_fp64": l2,

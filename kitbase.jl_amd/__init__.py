@@ -9,10 +9,11 @@ The directory name is not a Python identifier; load it as the package `kitbase_a
 
 Contents: solver.py (SolverSet, init_fvm, timestep, DeviceSolve, reconstruct/evolve/update/solve),
 phase.py, geometry.py, theory.py (setup restatements), _lib.py (ctypes over libkbgpu.so),
-serving.py (request loop: admission by footprint, continuous batching, p50/p99), configs.py
+serving.py (request loop: admission by footprint, continuous batching, p50/p99), autotune.py
+(schedule tuner), configs.py
 (the workloads), csrc/ (HIP kernels and the C ABI), build.py (hipcc recipe).
 """
-from . import _lib, configs, geometry, phase, serving, theory  # noqa: F401
+from . import _lib, autotune, configs, geometry, phase, serving, theory  # noqa: F401
 from .solver import (Context, ControlVolumes, DeviceSolve, Interfaces, SolverSet, batch_step, default_context,  # noqa: F401
                      evolve, init_fvm, make_desc, read_dict, reconstruct, solve, solve_bytes, timestep, update)
 

@@ -113,3 +113,36 @@ def test_serving_mix_is_reproducible():
     b = kb.configs.serving_requests(20, 50.0)
     assert [(k, t) for k, _, _, t in a] == [(k, t) for k, _, _, t in b]
     assert {k for k, *_ in kb.configs.serving_requests(200, 50.0)} == {"1d", "2d", "3v"}
+
+
+def test_autotuner_picks_the_fastest_and_caches(tmp_path):
+    """The tuner with a fake context: the candidate with the smallest step time wins and is cached."""
+    times = {(60, 4): 0.2, (60, 2): 0.15, (30, 4): 0.3}
+
+    class Ctx:
+        sched = (60, 4)
+
+        def set_schedule(self, a, b):
+            self.sched = (a, b)
+
+        def last_steps_ms(self):
+            return times[self.sched] * 6
+
+    class Dev:
+        def __init__(self, nv):
+            self.ks = kb.SolverSet(kb.configs.sod_1d1f1v(nu=nv))
+            self.precision = "fp32"
+
+    calls = []
+    orig = kb.solver.batch_step
+    kb.solver.batch_step = lambda devs, dts, n, ctx=None: calls.append(n)
+    try:
+        ctx = Ctx()
+        devs = [Dev(32), Dev(64)]
+        best, res, taken = kb.autotune.tune(ctx, devs, [1e-3, 1e-3], candidates=list(times), cache=tmp_path / "t.json")
+        assert best == (60, 2) and ctx.sched == (60, 2) and taken == sum(calls)
+        ctx.sched = (30, 4)
+        best2, _, taken2 = kb.autotune.tune(ctx, devs, [1e-3, 1e-3], candidates=list(times), cache=tmp_path / "t.json")
+        assert best2 == (60, 2) and taken2 == 0 and ctx.sched == (60, 2)
+    finally:
+        kb.solver.batch_step = orig
