@@ -49,7 +49,7 @@ template <class T> __global__ __launch_bounds__(256) void k_recon2d(const Solve2
 
 // evolve!  solver_evolution.jl:608-679: interior x-faces (i = 2..nx) then y-faces (j = 2..ny).
 // Wrappers flux_kfvs.jl:98-135 and flux_kcu.jl:149-191; UGKS uses the KCU wrapper's framing with
-// flux_ugks.jl:121-253.
+// flux_ugks.jl:121-253.  The KCU / UGKS face chain (FP64) comes from the records of k_face2d.
 template <class T, int FLUX> __global__ __launch_bounds__(256) void k_evolve2d(const Solve2D* __restrict__ pS) {
     __shared__ double red_s[4 * 10];
     const Solve2D& S = *pS;
@@ -121,18 +121,10 @@ template <class T, int FLUX> __global__ __launch_bounds__(256) void k_evolve2d(c
         fw[2] = r[2];
         fw[3] = 0.5 * (r[3] + r[4]);
     } else {
-        double wl[4], wr[4], WL[4], WR[4];
-        const double* swL = S.sw + (size_t)cL * 8 + dir * 4;
-        const double* swR = S.sw + (size_t)cR * 8 + dir * 4;
-        for (int m = 0; m < 4; m++) {
-            wl[m] = S.w[(size_t)cL * 4 + m] + dxL * swL[m];
-            wr[m] = S.w[(size_t)cR * 4 + m] - dxR * swR[m];
-        }
-        local_frame4(wl, n1, n2, WL);
-        local_frame4(wr, n1, n2, WR);
+        // the face chain was evaluated once per face by k_face2d
+        const double* rec = (dir == 0 ? S.xcoef : S.ycoef) + (size_t)fi * kFaceCoef;
         if (FLUX == KCU) {
-            KcuFace<4> F;
-            kcu_face<4>(WL, WR, K, S.gam, S.mu, S.om, dt, F);
+            const KcuFace<4>& F = *(const KcuFace<4>*)rec;
             const T A = T(F.prim[0] * (F.prim[3] / kPi)), lam = T(F.prim[3]), Ui = T(F.prim[1]), Vi = T(F.prim[2]);
             const T Mt1 = T(F.Mt1), Mt2 = T(F.Mt2), kb = T(K / (2.0 * F.prim[3]));
             KSum<T> a[5];
@@ -159,8 +151,7 @@ template <class T, int FLUX> __global__ __launch_bounds__(256) void k_evolve2d(c
             fw[2] = F.fw0[2] * len + F.Mt2 * r[2] * len;
             fw[3] = F.fw0[3] * len + F.Mt2 * 0.5 * (r[3] + r[4]) * len;
         } else {
-            UgksFace2 F;
-            ugks_face_2d(WL, WR, K, S.gam, S.mu, S.om, dt, dxL, dxR, F);
+            const UgksFace2& F = *(const UgksFace2*)rec;
             const T A = T(F.prim[0] * (F.prim[3] / kPi)), lam = T(F.prim[3]), Ui = T(F.prim[1]), Vi = T(F.prim[2]);
             const T kb = T(K / (2.0 * F.prim[3])), Mxi2 = T(F.Mxi2);
             const T Mt0 = T(F.Mt[0]), Mt1 = T(F.Mt[1]), Mt2 = T(F.Mt[2]), Mt3 = T(F.Mt[3]), Mt4 = T(F.Mt[4]);
@@ -426,6 +417,8 @@ hipError_t launch_recon2d(bool fp64, const Solve2D& S, const Solve2D* d_S, hipSt
 hipError_t launch_evolve2d(bool fp64, const Solve2D& S, const Solve2D* d_S, hipStream_t st) {
     const int nfaces = (S.nx - 1) * S.ny + S.nx * (S.ny - 1);
     if (nfaces > 0) {
+        hipError_t e0 = launch_face2d(false, S.flux, d_S, nullptr, 1, nfaces, st);
+        if (e0 != hipSuccess) return e0;
         dim3 g(nfaces), b(256);
         if (fp64) {
             if (S.flux == KFVS) hipLaunchKernelGGL((k_evolve2d<double, KFVS>), g, b, 0, st, d_S);

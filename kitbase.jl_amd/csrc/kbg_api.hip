@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -60,6 +61,10 @@ struct kbg_ctx {
     kbg::Pack* d_packs = nullptr;
     size_t packs_cap = 0;
     double* d_res = nullptr;  // [4096][5]
+    Solve2D* d_descs2 = nullptr;  // fused 2D launches: descriptors and cell / face prefix sums
+    size_t descs2_cap = 0;
+    int* d_pref2 = nullptr;
+    size_t pref2_cap = 0;
 };
 
 struct kbg_solve {
@@ -69,6 +74,7 @@ struct kbg_solve {
     size_t esz = 4;
     int nw = 3, N = 0, RS = 0;
     bool fused = false;   // stepped by the fused sweep (cell-fastest f)
+    int thr2 = 0;         // 2D: threads per workgroup of the fused step (0: phase kernels)
     int sweep_nw = 0;     // wavefronts per sweep workgroup
     double vtab_dt = -1;  // dt the velocity table was built for
     void* vtab = nullptr;
@@ -394,8 +400,26 @@ int ensure_phase_buffers(kbg_solve* s) {
                               hipMemcpyDeviceToDevice, s->ctx->stream));
         return push1d(s);
     }
-    return KBG_OK;
+    Solve2D& S = s->h2;
+    if (S.sf) return KBG_OK;
+    const size_t C = s->N, nv = s->d.nv;
+    struct {
+        void** dst;
+        size_t bytes;
+    } bufs[] = {
+        {(void**)&S.sw, C * 8 * 8},
+        {&S.sf, C * 4 * nv * s->esz},
+    };
+    for (auto& b : bufs) {
+        void* p;
+        if ((rc = dev_alloc(s, &p, b.bytes))) return rc;
+        HIPCHK(hipMemsetAsync(p, 0, b.bytes, s->ctx->stream));
+        *b.dst = p;
+    }
+    return push2d(s);
 }
+
+
 
 // boundary-face buffers follow the stored faces 1 and nx+1 after phase-level calls
 int sync_boundary_faces(kbg_solve* s) {
@@ -475,6 +499,8 @@ int kbg_ctx_destroy(kbg_ctx* c) {
     (void)hipFree(c->d_rpart);
     (void)hipFree(c->d_packs);
     (void)hipFree(c->d_res);
+    (void)hipFree(c->d_descs2);
+    (void)hipFree(c->d_pref2);
     for (auto& sd : c->side)
         if (sd) (void)hipStreamDestroy(sd);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -533,8 +559,12 @@ size_t kbg_solve_bytes(const kbg_desc* d) {
                N * 4 * nw * 8 + vt;
     }
     const size_t C = size_t(d->nx) * d->ny, F = size_t(d->nx + 1) * d->ny + size_t(d->nx) * (d->ny + 1);
-    return C * 2 * d->nv * esz * 3 + F * 2 * d->nv * esz + C * (2 * 4 * 8 + 8 * 8 + 16 * 8 + 16) + F * (4 + 3 + 4) * 8 +
-           3 * size_t(d->nv) * esz;
+    const size_t nv = d->nv;
+    // f, face fluxes, w / prim / residual rows / dx / dy, geometry + walls + fw, velocity grid
+    size_t b = C * 2 * nv * esz + F * 2 * nv * esz + C * (2 * 4 * 8 + 16 * 8 + 2 * 8) + F * (3 + 4 + 4) * 8 + 3 * nv * esz;
+    if (d->flux != KBG_KFVS) b += F * kbg::kFaceCoef * 8;
+    if (kbg::step2d_threads(d->precision == KBG_FP64, d->nv) == 0) b += C * 8 * 8 + C * 4 * nv * esz;  // slopes
+    return b;
 }
 
 int kbg_solve_create(kbg_ctx* ctx, const kbg_desc* desc, kbg_solve** out) {
@@ -647,15 +677,24 @@ int kbg_solve_create(kbg_ctx* ctx, const kbg_desc* desc, kbg_solve** out) {
             HIPCHK(hipMemset(dd, 0, FY * 4 * 8));
         }
         S.ywall = dd;
+        const bool coef = d.flux != KBG_KFVS;
         struct {
             void** dst;
             size_t bytes;
         } bufs[] = {
-            {(void**)&S.w, C * 4 * 8},          {(void**)&S.prim, C * 4 * 8},       {&S.f, C * 2 * d.nv * s->esz},
-            {(void**)&S.sw, C * 8 * 8},         {&S.sf, C * 4 * d.nv * s->esz},     {(void**)&S.xfw, FX * 4 * 8},
-            {&S.xff, FX * 2 * d.nv * s->esz},   {(void**)&S.yfw, FY * 4 * 8},       {&S.yff, FY * 2 * d.nv * s->esz},
+            {(void**)&S.w, C * 4 * 8},
+            {(void**)&S.prim, C * 4 * 8},
+            {&S.f, C * 2 * d.nv * s->esz},
+            {(void**)&S.xfw, FX * 4 * 8},
+            {&S.xff, FX * 2 * d.nv * s->esz},
+            {(void**)&S.yfw, FY * 4 * 8},
+            {&S.yff, FY * 2 * d.nv * s->esz},
+            {(void**)&S.xcoef, coef ? FX * kbg::kFaceCoef * 8 : 0},
+            {(void**)&S.ycoef, coef ? FY * kbg::kFaceCoef * 8 : 0},
+            {(void**)&s->rcell, C * 16 * 8},
         };
         for (auto& b : bufs) {
+            if (!b.bytes) continue;
             if ((rc = dev_alloc(s, &p, b.bytes))) return bail(rc);
             HIPCHK(hipMemset(p, 0, b.bytes));
             *b.dst = p;
@@ -665,13 +704,20 @@ int kbg_solve_create(kbg_ctx* ctx, const kbg_desc* desc, kbg_solve** out) {
         S.u = s->u;
         S.v = s->v;
         S.wt = s->wt;
-        if ((rc = dev_alloc(s, &p, C * 16 * 8))) return bail(rc);
-        s->rcell = (double*)p;
+        S.rcell = s->rcell;
+        // the fused step addresses f and the face fluxes with 32-bit byte offsets
+        s->thr2 = FX * 2 * d.nv * s->esz < (1ull << 31) && FY * 2 * d.nv * s->esz < (1ull << 31) ? kbg::step2d_threads(s->fp64, d.nv) : 0;
         if ((rc = dev_alloc(s, &p, 8 * 8))) return bail(rc);
         s->res = (double*)p;
         if ((rc = dev_alloc(s, &p, sizeof(Solve2D)))) return bail(rc);
         s->d2 = (Solve2D*)p;
-        if ((rc = push2d(s))) return bail(rc);
+        // slope buffers of the phase-level calls are allocated on first use, or now when the fused
+        // step cannot take this velocity grid
+        if (s->thr2 == 0) {
+            if ((rc = ensure_phase_buffers(s))) return bail(rc);
+        } else if ((rc = push2d(s))) {
+            return bail(rc);
+        }
     }
     *out = s;
     return KBG_OK;
@@ -732,6 +778,7 @@ int kbg_upload_slopes(kbg_solve* s, const double* sw, const double* sf) {
         if (sf && (rc = put_vec(s, s->h1.sf, sf, N * s->RS))) return rc;
         return KBG_OK;
     }
+    if ((rc = ensure_phase_buffers(s))) return rc;
     if (sw && (rc = put_f64(s, s->h2.sw, sw, N * 8))) return rc;
     if (sf && (rc = put_vec(s, s->h2.sf, sf, N * 4 * s->d.nv))) return rc;
     return KBG_OK;
@@ -747,6 +794,7 @@ int kbg_download_slopes(kbg_solve* s, double* sw, double* sf) {
         if (sf && (rc = get_vec(s, sf, s->h1.sf, N * s->RS))) return rc;
         return KBG_OK;
     }
+    if ((rc = ensure_phase_buffers(s))) return rc;
     if (sw && (rc = get_f64(s, sw, s->h2.sw, N * 8))) return rc;
     if (sf && (rc = get_vec(s, sf, s->h2.sf, N * 4 * s->d.nv))) return rc;
     return KBG_OK;
@@ -762,6 +810,7 @@ int kbg_upload_faces(kbg_solve* s, const double* fw, const double* ff) {
         if (ff && (rc = put_vec(s, s->h1.ff, ff, nf1 * s->RS))) return rc;
         return sync_boundary_faces(s);
     }
+    if ((rc = ensure_phase_buffers(s))) return rc;
     const size_t FX = size_t(s->d.nx + 1) * s->d.ny, FY = size_t(s->d.nx) * (s->d.ny + 1), RS = 2 * s->d.nv;
     if (fw) {
         if ((rc = put_f64(s, s->h2.xfw, fw, FX * 4))) return rc;
@@ -784,6 +833,7 @@ int kbg_download_faces(kbg_solve* s, double* fw, double* ff) {
         if (ff && (rc = get_vec(s, ff, s->h1.ff, nf1 * s->RS))) return rc;
         return KBG_OK;
     }
+    if ((rc = ensure_phase_buffers(s))) return rc;
     const size_t FX = size_t(s->d.nx + 1) * s->d.ny, FY = size_t(s->d.nx) * (s->d.ny + 1), RS = 2 * s->d.nv;
     if (fw) {
         if ((rc = get_f64(s, fw, s->h2.xfw, FX * 4))) return rc;
@@ -803,7 +853,7 @@ int kbg_reconstruct(kbg_solve* s) {
         if ((rc = ensure_phase_buffers(s)) || (rc = push1d(s))) return rc;
         HIPCHK(kbg::launch_recon1d(s->fp64, s->h1, s->d1, s->ctx->stream));
     } else {
-        if ((rc = push2d(s))) return rc;
+        if ((rc = ensure_phase_buffers(s)) || (rc = push2d(s))) return rc;
         HIPCHK(kbg::launch_recon2d(s->fp64, s->h2, s->d2, s->ctx->stream));
     }
     HIPCHK(hipStreamSynchronize(s->ctx->stream));
@@ -821,6 +871,7 @@ int kbg_evolve(kbg_solve* s, double dt) {
         HIPCHK(kbg::launch_wall1d(s->fp64, s->h1, s->d1, 0, 0, s->ctx->stream));
         if ((rc = sync_boundary_faces(s))) return rc;
     } else {
+        if ((rc = ensure_phase_buffers(s))) return rc;
         s->h2.dt = dt;
         if ((rc = push2d(s))) return rc;
         HIPCHK(kbg::launch_evolve2d(s->fp64, s->h2, s->d2, s->ctx->stream));
@@ -838,6 +889,7 @@ int kbg_update(kbg_solve* s, double dt, double* residual) {
         if ((rc = push1d(s))) return rc;
         rc = step_phase_1d(s, dt, residual);
     } else {
+        if ((rc = ensure_phase_buffers(s))) return rc;
         s->h2.dt = dt;
         if ((rc = push2d(s))) return rc;
         rc = step_phase_2d(s, dt, residual);
@@ -862,15 +914,74 @@ int kbg_batch_step(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double* 
     hipStream_t st = ctx->stream;
     int rc;
     // fused 1D solves go to the sweep; the rest are stepped phase by phase
-    std::vector<int> fused, others;
+    std::vector<int> fused, fused2, others;
     for (int i = 0; i < n; i++) {
         kbg_solve* s = solves[i];
         if (!s || s->ctx != ctx) return fail(KBG_ERR_ARG, "solve is NULL or belongs to another context");
         if (s->d.dim == 1 && s->fused) {
             fused.push_back(i);
             if ((rc = ensure_vtab(s, dts[i]))) return rc;
+        } else if (s->d.dim == 2 && s->thr2 > 0) {
+            fused2.push_back(i);
         } else {
             others.push_back(i);
+        }
+    }
+    // fused 2D solves: face-chain, face-flux and cell launches per (precision, flux, workgroup size)
+    // group and step, all on the context stream
+    struct Launch2 {
+        bool fp64;
+        int flux, threads, n, ncells, nfaces, ninterior;
+        size_t desc0, cpref0, fpref0, ipref0;
+    };
+    std::vector<Launch2> launches2;
+    std::vector<Solve2D> descs2;
+    std::vector<int> pref2;
+    {
+        std::map<std::tuple<int, int, int>, std::vector<int>> groups;
+        for (int i : fused2) groups[{solves[i]->fp64 ? 1 : 0, solves[i]->d.flux, solves[i]->thr2}].push_back(i);
+        for (auto& kv : groups) {
+            Launch2 L{std::get<0>(kv.first) == 1, std::get<1>(kv.first), std::get<2>(kv.first), (int)kv.second.size(), 0, 0, 0,
+                      descs2.size(), 0, 0, 0};
+            std::vector<int> cp{0}, fp{0}, ip{0};
+            for (int i : kv.second) {
+                kbg_solve* s = solves[i];
+                s->h2.dt = dts[i];
+                descs2.push_back(s->h2);
+                const long nx = s->d.nx, ny = s->d.ny;
+                const long nc = nx * ny, nf = (nx + 1) * ny + nx * (ny + 1), ni = (nx - 1) * ny + nx * (ny - 1);
+                if (cp.back() + nc > INT_MAX / 2 || fp.back() + nf > INT_MAX / 2) return fail(KBG_ERR_ARG, "2D batch too large for one launch");
+                cp.push_back(int(cp.back() + nc));
+                fp.push_back(int(fp.back() + nf));
+                ip.push_back(int(ip.back() + ni));
+            }
+            L.ncells = cp.back();
+            L.nfaces = fp.back();
+            L.ninterior = ip.back();
+            L.cpref0 = pref2.size();
+            pref2.insert(pref2.end(), cp.begin(), cp.end());
+            L.fpref0 = pref2.size();
+            pref2.insert(pref2.end(), fp.begin(), fp.end());
+            L.ipref0 = pref2.size();
+            pref2.insert(pref2.end(), ip.begin(), ip.end());
+            launches2.push_back(L);
+        }
+        if (!descs2.empty()) {
+            if (descs2.size() > ctx->descs2_cap) {
+                (void)hipFree(ctx->d_descs2);
+                ctx->d_descs2 = nullptr;
+                ctx->descs2_cap = descs2.size() * 2;
+                HIPCHK(hipMalloc((void**)&ctx->d_descs2, ctx->descs2_cap * sizeof(Solve2D)));
+            }
+            if (pref2.size() > ctx->pref2_cap) {
+                (void)hipFree(ctx->d_pref2);
+                ctx->d_pref2 = nullptr;
+                ctx->pref2_cap = pref2.size() * 2;
+                HIPCHK(hipMalloc((void**)&ctx->d_pref2, ctx->pref2_cap * sizeof(int)));
+            }
+            HIPCHK(hipMemcpyAsync(ctx->d_descs2, descs2.data(), descs2.size() * sizeof(Solve2D), hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync(ctx->d_pref2, pref2.data(), pref2.size() * sizeof(int), hipMemcpyHostToDevice, st));
+            HIPCHK(hipStreamSynchronize(st));
         }
     }
     const int seg = std::max(1, ctx->seg_cells);
@@ -982,6 +1093,12 @@ int kbg_batch_step(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double* 
             HIPCHK(kbg::launch_sweep1d(L.fp64, L.nf, L.flux, L.wg_waves, ctx->d_descs, ctx->d_items, ctx->d_packs + L.pack0,
                                        (int)L.npacks, par, (last && residuals) ? ctx->d_rpart : nullptr, sst[L.stream]));
         }
+        for (const Launch2& L : launches2) {
+            const Solve2D* dd = ctx->d_descs2 + L.desc0;
+            HIPCHK(kbg::launch_face2d(true, L.flux, dd, ctx->d_pref2 + L.ipref0, L.n, L.ninterior, st));
+            HIPCHK(kbg::launch_flux2d(L.fp64, L.flux, L.threads, dd, ctx->d_pref2 + L.fpref0, L.n, L.nfaces, st));
+            HIPCHK(kbg::launch_cell2d(L.fp64, L.threads, dd, ctx->d_pref2 + L.cpref0, L.n, L.ncells, (last && residuals) ? 1 : 0, st));
+        }
         for (int i : others) {
             kbg_solve* s = solves[i];
             double* r = (last && residuals) ? residuals + 5 * (size_t)i : nullptr;
@@ -1036,6 +1153,14 @@ int kbg_batch_step(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double* 
         kbg_solve* s = solves[i];
         s->h1.cur = (s->h1.cur + nsteps) & 1;
         if ((rc = push1d(s))) return rc;
+    }
+    // fused 2D solves: residuals from the last step's rows
+    if (residuals) {
+        for (int i : fused2) {
+            kbg_solve* s = solves[i];
+            HIPCHK(kbg::launch_resid2d(s->h2, s->rcell, s->res, st));
+            HIPCHK(hipMemcpyAsync(residuals + 5 * (size_t)i, s->res, 4 * 8, hipMemcpyDeviceToHost, st));
+        }
     }
     HIPCHK(hipStreamSynchronize(st));
     return KBG_OK;

@@ -61,8 +61,9 @@ template <> KBG_D double tdiv<double>(double a, double b) { return a / b; }
 // limiter evaluated per velocity; van Leer (the KitBase default) takes the fast division
 template <class T> KBG_D T limf(int kind, T sL, T sR) {
     if (kind == VANLEER) {
+        // (sign sL + sign sR) |sL| |sR| written without sign branches: sL |sR| + |sL| sR
         const T aL = fabs(sL), aR = fabs(sR);
-        return tdiv<T>((jsign(sL) + jsign(sR)) * aL * aR, aL + aR + T(1e-7));
+        return tdiv<T>(sL * aR + aL * sR, aL + aR + T(1e-7));
     }
     return limiter<T>(kind, sL, sR);
 }
@@ -334,6 +335,17 @@ KBG_D float texp(float x) { return __expf(x); }
 KBG_D double texp(double x) { return exp(x); }
 KBG_D float tsqrt(float x) { return sqrtf(x); }
 KBG_D double tsqrt(double x) { return sqrt(x); }
+
+// Raw buffer access: lane byte offset + uniform (scalar) offset.  Offsets past
+// the buffer read 0 and drop stores.  Word 3 of the resource is the GFX9 raw 32-bit data format.
+using rsrc_t = __amdgpu_buffer_rsrc_t;
+KBG_D rsrc_t make_rsrc(const void* p, unsigned bytes) { return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)bytes, 0x00020000); }
+KBG_D float bload(float*, rsrc_t r, int voff, int soff) { return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0)); }
+KBG_D double bload(double*, rsrc_t r, int voff, int soff) { return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0)); }
+KBG_D void bstore(float x, rsrc_t r, int voff, int soff) { __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), r, voff, soff, 0); }
+KBG_D void bstore(double x, rsrc_t r, int voff, int soff) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, x), r, voff, soff, 0);
+}
 
 // ---- reductions -------------------------------------------------------------------------------
 

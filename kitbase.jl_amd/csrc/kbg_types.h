@@ -65,13 +65,27 @@ struct Solve2D {
     double* w;                                     // [nx*ny][4]
     double* prim;
     void* f;                                       // [nx*ny][2][nv]
-    double* sw;                                    // [nx*ny][2 dirs][4]
-    void* sf;                                      // [nx*ny][2 dirs][2][nv]
+    double* sw;                                    // [nx*ny][2 dirs][4]          (phase-exact path)
+    void* sf;                                      // [nx*ny][2 dirs][2][nv]      (phase-exact path)
     double* xfw;                                   // [(nx+1)*ny][4]
     void* xff;                                     // [(nx+1)*ny][2][nv]
     double* yfw;
     void* yff;
+    double* xcoef;                                 // [(nx+1)*ny][kFaceCoef] KCU / UGKS face chain records
+    double* ycoef;                                 // [nx*(ny+1)][kFaceCoef]
+    double* rcell;                                 // [nx*ny][16] residual rows
 };
+
+// doubles per face-coefficient record (a UgksFace2 or a KcuFace<4>, padded)
+constexpr int kFaceCoef = 32;
+// Threads per workgroup of the fused 2D step (one face or cell per workgroup): 64 ceil(nv / 256), 4
+// velocities per thread, at most 1024 in FP32 and 512 in FP64 (the register budget of an FP64
+// thread); 0 when the velocity grid is larger (the phase kernels then step the solve)
+template <class T> constexpr int step2d_vpt() { return 4; }
+template <class T> constexpr int step2d_max_threads() { return sizeof(T) == 4 ? 1024 : 512; }
+constexpr int step2d_threads(bool fp64, int nv) {
+    return (nv + 255) / 256 * 64 <= (fp64 ? 512 : 1024) ? (nv + 255) / 256 * 64 : 0;
+}
 
 // One unit of work of the fused 1D sweep: cells [c0, c1] (physical, at most 60) of solve `solve`,
 // updated by nw wavefronts (1, 2, 4 or 8) that split the velocity grid evenly in multiples of 8.
@@ -116,5 +130,17 @@ hipError_t launch_recon2d(bool fp64, const Solve2D& S, const Solve2D* d_S, hipSt
 hipError_t launch_evolve2d(bool fp64, const Solve2D& S, const Solve2D* d_S, hipStream_t st);
 hipError_t launch_update2d(bool fp64, const Solve2D& S, const Solve2D* d_S, double* d_rcell, hipStream_t st);
 hipError_t launch_resid2d(const Solve2D& S, const double* d_rcell, double* d_res, hipStream_t st);
+// Face-chain records of the interior faces (KCU / UGKS; nothing for KFVS).  d_fstart: prefix sums of
+// interior-face counts over nsolves solves, or NULL for the single solve d_solves[0] with nfaces faces.
+// inline_sw rebuilds the conserved slopes from w instead of reading the stored sw.
+hipError_t launch_face2d(bool inline_sw, int flux, const Solve2D* d_solves, const int* d_fstart, int nsolves, int nfaces,
+                         hipStream_t st);
+// Fused 2D step, workgroups of `threads` (step2d_threads of every solve of the launch): face fluxes of
+// all faces (d_fstart: prefix sums of (nx+1) ny + nx (ny+1) per solve), then the cell update of all
+// cells (d_wg0: prefix sums of nx ny).  rows: write each solve's residual rows.
+hipError_t launch_flux2d(bool fp64, int flux, int threads, const Solve2D* d_solves, const int* d_fstart, int nsolves, int nfaces,
+                         hipStream_t st);
+hipError_t launch_cell2d(bool fp64, int threads, const Solve2D* d_solves, const int* d_wg0, int nsolves, int ncells, int rows,
+                         hipStream_t st);
 
 }  // namespace kbg

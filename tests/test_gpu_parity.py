@@ -217,6 +217,103 @@ def test_cavity_config_ugks(gpu_ctx, precision, tol):
     check_macro(cg.prim, co.prim, tol)
 
 
+@pytest.mark.parametrize("bc", [["maxwell", "maxwell", "fix", "fix"], ["fix", "fix", "maxwell", "maxwell"],
+                                ["maxwell", "fix", "fix", "maxwell"], ["fix", "fix", "fix", "fix"]])
+def test_cavity_mixed_boundaries_fp64(gpu_ctx, bc):
+    """Per-side boundaries, including stepped corner cells next to a :fix side, which read that
+    side's stored face.  KitBase initialises face fw to the cell state (struct_face.jl:96), which
+    would pour mass into those corners; the stored faces get small finite fluxes instead, so the
+    check sees them read."""
+    cfg = kb.configs.cavity_2d2f2v(flux="ugks", collision="shakhov", nx=9, ny=8, nu=10, nv=10, boundary=bc)
+
+    def stored_faces(ks, ctr, face):
+        face.fw[...] = 0.0
+        face.yfw[...] = 0.0
+        for row in (0, -1):
+            face.yfw[row, :, :] = [1e-6, 0.0, 0.0, 2e-6]
+            face.yff[row] = 1e-7 * ctr.f[row]
+
+    ks, cg, co = run_both(cfg, 6, "fp64", ctx=gpu_ctx, prepare=stored_faces)
+    assert np.all(np.isfinite(co.f))
+    assert rel_max(cg.f, co.f) <= 1e-9, rel_max(cg.f, co.f)
+    assert rel_max(cg.w, co.w) <= 1e-9
+
+
+@pytest.mark.parametrize("nu,nvv", [(4, 4), (7, 9), (16, 16), (24, 24), (33, 31), (48, 48)])
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-9), ("fp32", 1e-4)])
+def test_cavity_velocity_grids(gpu_ctx, nu, nvv, precision, tol):
+    """Velocity grids from 16 to 2304 points: one to sixteen wavefronts per cell (the largest FP64
+    grid is stepped by the phase kernels)."""
+    cfg = kb.configs.cavity_2d2f2v(nx=6, ny=7, nu=nu, nv=nvv, vMeshType="rectangle", collision="shakhov")
+    ks, cg, co = run_both(cfg, 5, precision, ctx=gpu_ctx)
+    if precision == "fp64":
+        assert rel_max(cg.f, co.f) <= tol, rel_max(cg.f, co.f)
+    check_macro(cg.prim, co.prim, tol)
+
+
+@pytest.mark.parametrize("over", [dict(interpOrder=1), dict(limiter="minmod"), dict(limiter="superbee"),
+                                  dict(limiter="vanalbaba"), dict(limiter="linear")])
+def test_cavity_reconstruction_variants_fp64(gpu_ctx, over):
+    cfg = kb.configs.cavity_2d2f2v(flux="kcu", nx=7, ny=6, nu=8, nv=8, **over)
+    ks, cg, co = run_both(cfg, 6, "fp64", ctx=gpu_ctx)
+    assert rel_max(cg.f, co.f) <= 1e-9, rel_max(cg.f, co.f)
+
+
+@pytest.mark.parametrize("flux", ["kfvs", "ugks"])
+def test_cavity_fused_and_phase_paths_agree_fp32(gpu_ctx, flux):
+    """The fused 2D step and the phase-level calls (reconstruct!, evolve!, update!) on the config 3
+    grid: same arithmetic up to FP32 summation order."""
+    cfg = kb.configs.cavity_2d2f2v(flux=flux, nx=24, ny=20)
+    _, c1, _ = run_both(cfg, 10, "fp32", fused=True, ctx=gpu_ctx)
+    _, c2, _ = run_both(cfg, 10, "fp32", fused=False, ctx=gpu_ctx)
+    assert rel_max(c1.f, c2.f) <= 1e-5, rel_max(c1.f, c2.f)
+    assert rel_max(c1.w, c2.w) <= 1e-6, rel_max(c1.w, c2.w)
+
+
+def test_cavity_mass_conserved_fp64(gpu_ctx):
+    """Maxwell walls carry no mass: total mass sum(rho dx dy) is constant to round-off."""
+    ks = kb.SolverSet(kb.configs.cavity_2d2f2v(nx=20, ny=20, nu=16, nv=16, knudsen=0.075))
+    ctr, face = kb.init_fvm(ks)
+    dt = kb.timestep(ks, ctr)
+    dev = kb.DeviceSolve(ks, ctr, face, precision="fp64", ctx=gpu_ctx)
+    area = ks.ps.dx * ks.ps.dy
+    m0 = float(np.sum(ctr.w[..., 0] * area))
+    c = copy.deepcopy(ctr)
+    for _ in range(4):
+        dev.step(dt, 50)
+        dev.download(c)
+        assert abs(float(np.sum(c.w[..., 0] * area)) - m0) <= 1e-12 * m0
+    dev.close()
+
+
+def test_cavity_batch_equals_individual(gpu_ctx):
+    """Several 2D solves of different shapes, precisions and fluxes in one batch_step give the same
+    bits as each solve stepped alone."""
+    cfgs = [kb.configs.cavity_2d2f2v(nx=12, ny=10, nu=8, nv=8, flux="ugks"),
+            kb.configs.cavity_2d2f2v(nx=9, ny=14, nu=16, nv=12, flux="ugks", collision="shakhov"),
+            kb.configs.cavity_2d2f2v(nx=8, ny=8, nu=24, nv=24, flux="kfvs", vMeshType="rectangle"),
+            kb.configs.cavity_2d2f2v(nx=10, ny=7, nu=10, nv=10, flux="kcu")]
+    precs = ["fp32", "fp32", "fp32", "fp64"]
+    alone, batch, dts, ctrs = [], [], [], []
+    for cfg, p in zip(cfgs, precs):
+        ks = kb.SolverSet(cfg)
+        ctr, face = kb.init_fvm(ks)
+        dt = kb.timestep(ks, ctr)
+        alone.append(kb.DeviceSolve(ks, ctr, face, p, gpu_ctx))
+        batch.append(kb.DeviceSolve(ks, ctr, face, p, gpu_ctx))
+        dts.append(dt)
+        ctrs.append(ctr)
+    kb.batch_step(batch, dts, 7, ctx=gpu_ctx)
+    for a, b, dt, ctr in zip(alone, batch, dts, ctrs):
+        a.step(dt, 7)
+        c1, c2 = copy.deepcopy(ctr), copy.deepcopy(ctr)
+        a.download(c1)
+        b.download(c2)
+        assert np.array_equal(c1.f, c2.f) and np.array_equal(c1.w, c2.w)
+        a.close()
+        b.close()
+
+
 def test_cavity_phase_exact(gpu_ctx):
     ks = kb.SolverSet(kb.configs.cavity_2d2f2v(nx=8, ny=9, nu=10, nv=10, flux="ugks", collision="shakhov"))
     ctr, face = kb.init_fvm(ks)
