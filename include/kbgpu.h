@@ -121,6 +121,12 @@ int kbg_evolve(kbg_solve *s, double dt);
  * update_boundary.jl:6-306.  residual: [nw] or NULL. */
 int kbg_update(kbg_solve *s, double dt, double *residual);
 
+/* timestep(KS, ctr, simTime)            solver_main.jl:240-342, from the solve's current prim on the
+ * device: dt = cfl / max over physical cells of (max(u1, |U|) + c) / dx (2D: + (max(v1, |V|) + c) / dy),
+ * c = (gamma / (2 lambda))^(1/2) (theory_macro.jl:421), then min(dt, max_time - sim_time).  u1, v1 are
+ * the velocity-space bounds VSpace.u1 / v1 (v1 unused in 1D). */
+int kbg_timestep(kbg_solve *s, double cfl, double u1, double v1, double sim_time, double max_time, double *dt);
+
 /* nsteps iterations of solve!'s loop body (solver_main.jl:32-51) with a fixed dt, fused on the
  * device; residual (of the last step) may be NULL.  Slopes and face arrays are not kept. */
 int kbg_step(kbg_solve *s, double dt, int nsteps, double *residual);

@@ -69,6 +69,7 @@ _PROTOS = {
     "kbg_reconstruct": (C.c_int, [C.c_void_p]),
     "kbg_evolve": (C.c_int, [C.c_void_p, C.c_double]),
     "kbg_update": (C.c_int, [C.c_void_p, C.c_double, _dp]),
+    "kbg_timestep": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, _dp]),
     "kbg_step": (C.c_int, [C.c_void_p, C.c_double, C.c_int, _dp]),
     "kbg_batch_step": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_int, _dp, C.c_int, _dp]),
 }

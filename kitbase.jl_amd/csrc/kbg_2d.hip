@@ -406,6 +406,37 @@ __global__ void k_resid2(const double* __restrict__ rows, int nrows, double ncel
     }
 }
 
+// timestep (solver_main.jl:240-342): tmax = max over the cells of (max(u1, |U|) + c) / dx, plus
+// (max(v1, |V|) + c) / dy in 2D, with the sound speed c = (gamma / (2 lambda))^(1/2)
+// (theory_macro.jl:421).  A maximum does not depend on the order it is taken in, so the result is
+// the same on every run.
+__global__ __launch_bounds__(256) void k_cfl(const double* __restrict__ prim, int nw, int k0, int n, const double* __restrict__ dx,
+                                             const double* __restrict__ dy, double u1, double v1, double gam,
+                                             double* __restrict__ out) {
+    __shared__ double part[256];
+    double m = 0.0;
+    for (int k = threadIdx.x; k < n; k += 256) {
+        const double* p = prim + (size_t)(k0 + k) * nw;
+        const double sos = sqrt(0.5 * gam / p[nw - 1]);
+        double t = (fmax(u1, fabs(p[1])) + sos) / dx[k0 + k];
+        if (dy) t += (fmax(v1, fabs(p[2])) + sos) / dy[k0 + k];
+        m = fmax(m, t);
+    }
+    part[threadIdx.x] = m;
+    __syncthreads();
+    for (int h = 128; h >= 1; h >>= 1) {
+        if (threadIdx.x < h) part[threadIdx.x] = fmax(part[threadIdx.x], part[threadIdx.x + h]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = part[0];
+}
+
+hipError_t launch_cfl(const double* d_prim, int nw, int k0, int n, const double* d_dx, const double* d_dy, double u1, double v1,
+                      double gam, double* d_out, hipStream_t st) {
+    hipLaunchKernelGGL(k_cfl, dim3(1), dim3(256), 0, st, d_prim, nw, k0, n, d_dx, d_dy, u1, v1, gam, d_out);
+    return hipGetLastError();
+}
+
 hipError_t launch_recon2d(bool fp64, const Solve2D& S, const Solve2D* d_S, hipStream_t st) {
     if (!S.interp2) return hipSuccess;
     const int n = S.nx * S.ny;

@@ -899,6 +899,25 @@ int kbg_update(kbg_solve* s, double dt, double* residual) {
     return KBG_OK;
 }
 
+int kbg_timestep(kbg_solve* s, double cfl, double u1, double v1, double sim_time, double max_time, double* dt) {
+    if (!s || !dt) return fail(KBG_ERR_ARG, "NULL argument");
+    if (!(cfl > 0.0)) return fail(KBG_ERR_ARG, "cfl must be positive");
+    hipStream_t st = s->ctx->stream;
+    if (s->d.dim == 1) {
+        const Solve1D& S = s->h1;
+        HIPCHK(kbg::launch_cfl(S.prim, s->nw, s->d.ng, s->d.nx, s->dx, nullptr, u1, 0.0, s->d.gamma, s->res, st));
+    } else {
+        const Solve2D& S = s->h2;
+        HIPCHK(kbg::launch_cfl(S.prim, 4, 0, s->N, s->dx, s->dy, u1, v1, s->d.gamma, s->res, st));
+    }
+    double tmax = 0.0;
+    HIPCHK(hipMemcpyAsync(&tmax, s->res, 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    const double d = cfl / tmax;
+    *dt = d < (max_time - sim_time) ? d : max_time - sim_time;
+    return KBG_OK;
+}
+
 int kbg_step(kbg_solve* s, double dt, int nsteps, double* residual) {
     if (!s) return fail(KBG_ERR_ARG, "NULL solve");
     if (nsteps < 0) return fail(KBG_ERR_ARG, "nsteps < 0");

@@ -130,6 +130,9 @@ hipError_t launch_recon2d(bool fp64, const Solve2D& S, const Solve2D* d_S, hipSt
 hipError_t launch_evolve2d(bool fp64, const Solve2D& S, const Solve2D* d_S, hipStream_t st);
 hipError_t launch_update2d(bool fp64, const Solve2D& S, const Solve2D* d_S, double* d_rcell, hipStream_t st);
 hipError_t launch_resid2d(const Solve2D& S, const double* d_rcell, double* d_res, hipStream_t st);
+// max over cells k0 .. k0+n-1 of the CFL rate (kbg_2d.hip k_cfl); d_dy NULL in 1D
+hipError_t launch_cfl(const double* d_prim, int nw, int k0, int n, const double* d_dx, const double* d_dy, double u1, double v1,
+                      double gam, double* d_out, hipStream_t st);
 // Face-chain records of the interior faces (KCU / UGKS; nothing for KFVS).  d_fstart: prefix sums of
 // interior-face counts over nsolves solves, or NULL for the single solve d_solves[0] with nfaces faces.
 // inline_sw rebuilds the conserved slopes from w instead of reading the stored sw.

@@ -148,6 +148,15 @@ function KitBase.update!(KS::SolverSet, dev::DeviceSolve, face, dt, residual; co
     return nothing
 end
 
+# timestep(KS, ctr, simTime) (solver_main.jl:240-342) from the device-resident prim
+function KitBase.timestep(KS::SolverSet, dev::DeviceSolve, simTime=0.0)
+    v1 = hasproperty(KS.vs, :v1) ? KS.vs.v1 : 0.0
+    dt = Ref{Cdouble}(0.0)
+    check(ccall((:kbg_timestep, libkbgpu), Cint, (Ptr{Cvoid}, Cdouble, Cdouble, Cdouble, Cdouble, Cdouble, Ref{Cdouble}),
+                dev.handle, KS.set.cfl, KS.vs.u1, v1, simTime, KS.set.maxTime, dt))
+    return dt[]
+end
+
 """
     step!(dev, dt, nsteps) -> residual
 

@@ -67,7 +67,9 @@ class DeviceBackend:
 
     def create(self, prepared, req: Request):
         ks, ctr, face, dt = prepared
-        return self.kb.DeviceSolve(ks, ctr, face, precision=req.precision, ctx=self.ctx), dt
+        # the face arrays hold no state the fused step reads (boundary fluxes are recomputed every
+        # step), so only the cells are uploaded and no phase-level buffers are allocated
+        return self.kb.DeviceSolve(ks, ctr, None, precision=req.precision, ctx=self.ctx), dt
 
     def step(self, handles, dts, nsteps: int):
         self.kb.batch_step(handles, dts, nsteps, ctx=self.ctx)

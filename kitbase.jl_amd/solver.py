@@ -567,6 +567,14 @@ class DeviceSolve:
     def update(self, dt: float, residual: np.ndarray | None = None):
         _lib.check(self.lib.kbg_update(self.handle, dt, _lib.dptr(residual) if residual is not None else None))
 
+    def timestep(self, simTime: float = 0.0) -> float:
+        """timestep(KS, ctr, simTime) (solver_main.jl:240-342) from the device-resident prim."""
+        ks = self.ks
+        v1 = ks.vs.v1 if ks.dim == 2 else 0.0
+        dt = C.c_double()
+        _lib.check(self.lib.kbg_timestep(self.handle, ks.set.cfl, ks.vs.u1, v1, simTime, ks.set.maxTime, C.byref(dt)))
+        return dt.value
+
     def step(self, dt: float, nsteps: int = 1, residual: np.ndarray | None = None):
         r = None
         if residual is not None:

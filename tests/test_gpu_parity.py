@@ -384,3 +384,72 @@ def test_residual_fused_matches_oracle(gpu_ctx):
     rg = np.zeros(3)
     dev.step(dt, 5, rg)
     assert np.allclose(rg, ro[:3], rtol=1e-8)
+
+
+# ---------------------------------------------------------------- timestep and checkpoint / resume
+
+_TS_CASES = {
+    "sod": lambda: kb.configs.sod_1d1f1v(),
+    "sod_ghost": lambda: kb.configs.sod_1d1f1v(nxg=2, nx=60),
+    "shock_2f": lambda: kb.configs.shock_1d2f1v(nx=80, nu=48),
+    "shock_3v": lambda: kb.configs.shock_1d1f3v(nx=30, nu=12, nv=12, nw=12),
+    "cavity": lambda: kb.configs.cavity_2d2f2v(nx=16, ny=12, nu=12, nv=12),
+}
+
+
+@pytest.mark.parametrize("case", sorted(_TS_CASES))
+def test_device_timestep_matches_host(gpu_ctx, case):
+    """timestep (solver_main.jl:240-342) from the device-resident prim equals the host one on the same
+    state, before and after steps, and clips to maxTime - simTime."""
+    ks = kb.SolverSet(_TS_CASES[case]())
+    ctr, face = kb.init_fvm(ks)
+    dt = kb.timestep(ks, ctr)
+    dev = kb.DeviceSolve(ks, ctr, face, "fp64", gpu_ctx)
+    assert dev.timestep() == pytest.approx(dt, rel=1e-15, abs=0.0)
+    dev.step(dt, 20)
+    c = copy.deepcopy(ctr)
+    dev.download(c)
+    dt2 = kb.timestep(ks, c)
+    assert dt2 != dt
+    assert dev.timestep() == pytest.approx(dt2, rel=1e-15, abs=0.0)
+    sim = ks.set.maxTime - 0.5 * dt2
+    assert dev.timestep(sim) == ks.set.maxTime - sim
+    dev.close()
+
+
+_CK_CASES = {
+    "sod_fp32": (lambda: kb.configs.sod_1d1f1v(), "fp32"),
+    "shock_ugks_fp32": (lambda: kb.configs.shock_1d2f1v(nx=120), "fp32"),
+    "walls_fp64": (lambda: kb.configs.shock_1d2f1v(boundary="maxwell", nx=60, nu=40, flux="kfvs"), "fp64"),
+    "cavity_fp32": (lambda: kb.configs.cavity_2d2f2v(nx=20, ny=18, nu=16, nv=16), "fp32"),
+    "cavity_fp64": (lambda: kb.configs.cavity_2d2f2v(nx=12, ny=10, nu=12, nv=12, collision="shakhov"), "fp64"),
+}
+
+
+@pytest.mark.parametrize("case", sorted(_CK_CASES))
+def test_checkpoint_resume_bitwise(gpu_ctx, case, tmp_path):
+    """download (before write_jld, solver_main.jl:59-61) -> file -> upload into a new solve (as after
+    initialize("*.jld2"), solver_init.jl:19-26) -> continue: the resumed solve matches the
+    uninterrupted one bit for bit."""
+    cfgf, prec = _CK_CASES[case]
+    ks = kb.SolverSet(cfgf())
+    ctr, face = kb.init_fvm(ks)
+    dt = kb.timestep(ks, ctr)
+    a = kb.DeviceSolve(ks, ctr, face, prec, gpu_ctx)
+    a.step(dt, 9)
+    snap = copy.deepcopy(ctr)
+    a.download(snap)
+    path = tmp_path / "checkpoint.npz"
+    np.savez(path, w=snap.w, prim=snap.prim, f=snap.f)
+    z = np.load(path)
+    back = copy.deepcopy(ctr)
+    back.w[...], back.prim[...], back.f[...] = z["w"], z["prim"], z["f"]
+    b = kb.DeviceSolve(ks, back, None, prec, gpu_ctx)
+    a.step(dt, 13)
+    b.step(dt, 13)
+    ca, cb = copy.deepcopy(ctr), copy.deepcopy(ctr)
+    a.download(ca)
+    b.download(cb)
+    assert np.array_equal(ca.f, cb.f) and np.array_equal(ca.w, cb.w) and np.array_equal(ca.prim, cb.prim)
+    a.close()
+    b.close()
