@@ -79,6 +79,35 @@ def traffic_from_profiles(points):
     return vals["hbm_bytes_per_step"]
 
 
+def cavity_leg(kb, ctx, steps):
+    """Config 3 geometry (64^2 cells x 32^2 Maxwell-quadrature velocities, UGKS + BGK, Maxwell walls)
+    stepped by the fused 2D step in FP32 and FP64; device time from the library's step events."""
+    cavity = {"config": "2D2F2V cavity 64^2 x 32^2, UGKS + BGK, Kn 0.075, Maxwell walls", "steps": steps,
+              "kernels": "k_face2d (FP64 face chain) + k_flux2d (face fluxes, inline slopes) + k_cell2d (update)",
+              "bound": "valu: the UGKS face flux is about 100 FP32 operations per face and velocity"}
+    for prec in ("fp32", "fp64"):
+        ks = kb.SolverSet(kb.configs.cavity_2d2f2v())
+        ctr, face = kb.init_fvm(ks)
+        dt = kb.timestep(ks, ctr)
+        d = kb.DeviceSolve(ks, ctr, face, precision=prec, ctx=ctx)
+        d.step(dt, 2)
+        ctx.sync()
+        t1 = time.perf_counter()
+        d.step(dt, steps)
+        wall_c = time.perf_counter() - t1
+        dev_ms = ctx.last_steps_ms() / steps
+        d.close()
+        pts = ks.ps.nx * ks.ps.ny * ks.vs.nv
+        esz = 4 if prec == "fp32" else 8
+        alg = 2 * 2 * esz * pts + 2 * 4 * 8 * ks.ps.nx * ks.ps.ny  # f (h, b) read + write, w read + write
+        cavity[prec] = {"updates_per_s": pts * steps / wall_c, "ms_per_step": wall_c * 1e3 / steps,
+                        "device_ms_per_step": dev_ms, "algorithmic_bytes_per_step": alg,
+                        "hbm_frac": alg / (dev_ms * 1e-3) / (HBM_PEAK_GBS * 1e9)}
+    log(f"[bench] cavity: fp32 {cavity['fp32']['device_ms_per_step']:.3f} ms/step, "
+        f"fp64 {cavity['fp64']['device_ms_per_step']:.3f} ms/step (device)")
+    return cavity
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,6 +120,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 single-solve comparison")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU work of the baseline sample")
     ap.add_argument("--cavity-steps", type=int, default=20, help="timed steps of the 2D cavity leg (0 skips it)")
+    ap.add_argument("--cavity-only", action="store_true", help="run only the 2D cavity leg (profiling)")
     ap.add_argument("--serve", type=int, default=48, metavar="N",
                     help="serving leg: N requests of the mixed stream (0 skips it)")
     ap.add_argument("--serve-rate", type=float, default=100.0, help="Poisson arrival rate of the serving leg (1/s)")
@@ -124,6 +154,9 @@ def main():
     from loader import load_package
 
     kb = load_package()
+    if a.cavity_only:
+        print(json.dumps({"cavity_2d": cavity_leg(kb, kb.Context(0), max(1, a.cavity_steps))}), flush=True)
+        return
     K, W = a.steps, a.warmup
     t0 = time.time()
     inst = build_instances(kb, a.instances)
@@ -214,23 +247,8 @@ def main():
         log(f"[bench] worst instance {worst_case[0]}: errors {worst_case[1]}, nx {worst_case[2]}, nv {worst_case[3]}, "
             f"{worst_case[4]}, Kn {worst_case[5]:.3e}, max|U| {worst_case[6]:.3e}, max|dU| {worst_case[7]:.3e}")
 
-    # ---- 2D lid-driven cavity (64^2 cells x 32^2 velocities, UGKS, Maxwell walls), phase by phase
-    cavity = None
-    if a.cavity_steps:
-        cavity = {"config": "2D2F2V cavity 64^2 x 32^2, UGKS + BGK, Kn 0.075", "steps": a.cavity_steps}
-        for prec in ("fp32", "fp64"):
-            ks = kb.SolverSet(kb.configs.cavity_2d2f2v())
-            ctr, face = kb.init_fvm(ks)
-            dt = kb.timestep(ks, ctr)
-            d = kb.DeviceSolve(ks, ctr, face, precision=prec, ctx=ctx)
-            d.step(dt, 2)
-            t1 = time.perf_counter()
-            d.step(dt, a.cavity_steps)
-            wall_c = time.perf_counter() - t1
-            d.close()
-            pts = ks.ps.nx * ks.ps.ny * ks.vs.nv
-            cavity[prec] = {"updates_per_s": pts * a.cavity_steps / wall_c, "ms_per_step": wall_c * 1e3 / a.cavity_steps}
-        log(f"[bench] cavity: fp32 {cavity['fp32']['ms_per_step']:.3f} ms/step, fp64 {cavity['fp64']['ms_per_step']:.3f} ms/step")
+    # ---- 2D lid-driven cavity (64^2 cells x 32^2 velocities, UGKS, Maxwell walls), fused step
+    cavity = cavity_leg(kb, ctx, a.cavity_steps) if a.cavity_steps else None
 
     # ---- serving leg: Poisson arrivals of 1D tubes, 2D cavities and 1D1F3V requests, served together
     serve = None

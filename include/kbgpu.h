@@ -88,6 +88,11 @@ int kbg_ctx_sync(kbg_ctx *ctx);
  * concurrent streams a batch spreads its kernel groups over (1..4; 0 = all 4). */
 int kbg_ctx_set_schedule(kbg_ctx *ctx, int seg_cells, int streams);
 int kbg_ctx_get_schedule(kbg_ctx *ctx, int *seg_cells, int *streams);
+/* Replay of batch steps through HIP graphs (default off): kbg_batch_step / kbg_step calls without
+ * residuals capture the launches of a step pair once per batch composition and replay them; off
+ * launches every kernel one by one.  Same results either way; graphs cut host launch work when the
+ * host, not the device, bounds the step rate.  No counterpart in KitBase. */
+int kbg_ctx_set_graphs(kbg_ctx *ctx, int enable);
 /* HIP events on the context stream, for timing: record slot (0..15), elapsed ms between slots.
  * kbg_step / kbg_batch_step record slots KBG_EV_STEPS_BEGIN and KBG_EV_STEPS_END around their
  * time-step loop (after the descriptor upload, before the residual download), so the elapsed

@@ -23,11 +23,11 @@ template <class T> struct FRow {
 template <class T> KBG_D FRow<T> frow(const Solve1D& S, T* f, int k) { return FRow<T>{f + (long)k * S.fcs, S.nv, S.fqs, S.fvs}; }
 
 // ------------------------------------------------------------------------------------------------
-// phase-exact kernels (one workgroup of 256 threads per cell or face)
+// phase-exact kernels (one workgroup of 256 threads per cell or face, 1024 for large velocity grids)
 // ------------------------------------------------------------------------------------------------
 
 // reconstruct!  solver_reconstruction.jl:56-153: slopes of the cells slope_lo..slope_hi
-template <class T> __global__ __launch_bounds__(256) void k_recon1d(const Solve1D* __restrict__ pS) {
+template <class T> __global__ __launch_bounds__(1024) void k_recon1d(const Solve1D* __restrict__ pS) {
     const Solve1D& S = *pS;
     const int i = S.slope_lo + blockIdx.x;
     const int k = i - 1 + S.ng, nw = S.nw, RS = S.nf * S.nv;
@@ -50,9 +50,9 @@ template <class T> __global__ __launch_bounds__(256) void k_recon1d(const Solve1
 
 // evolve!  solver_evolution.jl:39-89 over the faces face_lo..face_hi (flux kernels flux_kfvs.jl,
 // flux_kcu.jl, flux_ugks.jl; 1F1V, 2F1V and 1F3V)
-template <class T, int NF, int VD, int FLUX> __global__ __launch_bounds__(256) void k_evolve1d(const Solve1D* __restrict__ pS) {
+template <class T, int NF, int VD, int FLUX> __global__ __launch_bounds__(1024) void k_evolve1d(const Solve1D* __restrict__ pS) {
     constexpr int NW = VD == 3 ? 5 : 3;
-    __shared__ double red_s[4 * 8];
+    __shared__ double red_s[16 * 8];
     const Solve1D& S = *pS;
     const int k = S.face_lo + blockIdx.x;  // physical face
     const int ng = S.ng, nv = S.nv, RS = NF * nv;
@@ -105,7 +105,7 @@ template <class T, int NF, int VD, int FLUX> __global__ __launch_bounds__(256) v
             }
         }
         double r[5] = {acc[0].val(), acc[1].val(), acc[2].val(), acc[3].val(), acc[4].val()};
-        block_allreduce256<5>(r, red_s);
+        block_allreduce<5>(r, red_s);
         if (threadIdx.x == 0) {
             if (VD == 1) {
                 fw[0] = r[0];
@@ -156,7 +156,7 @@ template <class T, int NF, int VD, int FLUX> __global__ __launch_bounds__(256) v
             }
         }
         double r[5] = {acc[0].val(), acc[1].val(), acc[2].val(), acc[3].val(), acc[4].val()};
-        block_allreduce256<5>(r, red_s);
+        block_allreduce<5>(r, red_s);
         if (threadIdx.x == 0) {
             if (VD == 1) {
                 fw[0] = F.fw0[0] + F.Mt2 * r[0];
@@ -206,7 +206,7 @@ template <class T, int NF, int VD, int FLUX> __global__ __launch_bounds__(256) v
             ff[o] = Mt0 * x * B + Mt1 * uu * (d ? pLB : pRB) + Mt2 * x * pTB + Mt3 * x * b - Mt4 * uu * sbv;
         }
         double r[6] = {acc[0].val(), acc[1].val(), acc[2].val(), acc[3].val(), acc[4].val(), acc[5].val()};
-        block_allreduce256<6>(r, red_s);
+        block_allreduce<6>(r, red_s);
         if (threadIdx.x == 0) {
             fw[0] = F.fw0[0] + (F.Mt[3] * r[0] - F.Mt[4] * r[1]);
             fw[1] = F.fw0[1] + (F.Mt[3] * r[2] - F.Mt[4] * r[3]);
@@ -218,8 +218,8 @@ template <class T, int NF, int VD, int FLUX> __global__ __launch_bounds__(256) v
 // Maxwell diffuse wall, 1D2F1V   flux_boundary.jl:39-61 via evolve_boundary.jl:4-47.
 // blockIdx.x = 0: face 1 from cell 1 (rot = 1); 1: face nx+1 from cell nx (rot = -1).
 // fused != 0 writes the boundary-face buffers (reading f[cur ^ par]) instead of fw/ff.
-template <class T> __global__ __launch_bounds__(256) void k_wall1d(const Solve1D* __restrict__ pS, int fused, int par) {
-    __shared__ double red_s[4 * 8];
+template <class T> __global__ __launch_bounds__(1024) void k_wall1d(const Solve1D* __restrict__ pS, int fused, int par) {
+    __shared__ double red_s[16 * 8];
     const Solve1D& S = *pS;
     const int side = blockIdx.x;
     if ((side == 0 && !S.wallL) || (side == 1 && !S.wallR)) return;
@@ -245,7 +245,7 @@ template <class T> __global__ __launch_bounds__(256) void k_wall1d(const Solve1D
         else a1.add(wq * x * texp(-lw * (c * c)));
     }
     double r[2] = {a0.val(), a1.val()};
-    block_allreduce256<2>(r, red_s);
+    block_allreduce<2>(r, red_s);
     const double SG = pow(bc[2] / kPi, 0.5) * r[1];
     const double rho = -r[0] / SG;
     const T A = T(rho * sqrt(bc[2] / kPi)), kb = T(K / (2.0 * bc[2])), dtT = T(dt);
@@ -265,7 +265,7 @@ template <class T> __global__ __launch_bounds__(256) void k_wall1d(const Solve1D
         ff[nv + iv] = x * bW * dtT;
     }
     double s[4] = {m0.val(), m1.val(), m2.val(), m3.val()};
-    block_allreduce256<4>(s, red_s);
+    block_allreduce<4>(s, red_s);
     if (threadIdx.x == 0) {
         fw[0] = s[0] * dt;
         fw[1] = s[1] * dt;
@@ -276,9 +276,9 @@ template <class T> __global__ __launch_bounds__(256) void k_wall1d(const Solve1D
 // update!  solver_update.jl:61-93 + update_boundary.jl:6-61 over the cells upd_lo..upd_hi; the
 // boundary cells 1 and nx are stepped with BGK (update_boundary! does not forward `coll`).
 // step kernels step_1f.jl:6-55 (1F1V), :62-115 (1F3V), step_2f.jl:6-64 (2F1V)
-template <class T, int NF, int VD> __global__ __launch_bounds__(256) void k_update1d(const Solve1D* __restrict__ pS, double* __restrict__ rcell) {
+template <class T, int NF, int VD> __global__ __launch_bounds__(1024) void k_update1d(const Solve1D* __restrict__ pS, double* __restrict__ rcell) {
     constexpr int NW = VD == 3 ? 5 : 3;
-    __shared__ double red_s[4 * 8];
+    __shared__ double red_s[16 * 8];
     const Solve1D& S = *pS;
     const int c = S.upd_lo + blockIdx.x;
     const int ng = S.ng, nv = S.nv, RS = NF * nv, nx = S.nx;
@@ -311,7 +311,7 @@ template <class T, int NF, int VD> __global__ __launch_bounds__(256) void k_upda
             if (NF == 2) a.add(WT[iv] * cc * f[nv + iv]);
         }
         double r[1] = {a.val()};
-        block_allreduce256<1>(r, red_s);
+        block_allreduce<1>(r, red_s);
         q = 0.5 * r[0];
     }
     double wn[NW], pn[NW];
@@ -395,18 +395,22 @@ __global__ __launch_bounds__(256) void k_resid(const double* __restrict__ rows, 
     }
 }
 
+// Workgroup size of the phase kernels: 256 threads, or 1024 for large velocity grids (1D1F3V), whose
+// few cells and faces would otherwise leave each thread a long serial loop
+static int phase_threads(const Solve1D& S) { return S.nv >= 2048 ? 1024 : 256; }
+
 hipError_t launch_recon1d(bool fp64, const Solve1D& S, const Solve1D* d_S, hipStream_t st) {
     const int n = S.slope_hi - S.slope_lo + 1;
     if (n <= 0 || !S.interp2) return hipSuccess;
-    if (fp64) hipLaunchKernelGGL(k_recon1d<double>, dim3(n), dim3(256), 0, st, d_S);
-    else hipLaunchKernelGGL(k_recon1d<float>, dim3(n), dim3(256), 0, st, d_S);
+    if (fp64) hipLaunchKernelGGL(k_recon1d<double>, dim3(n), dim3(phase_threads(S)), 0, st, d_S);
+    else hipLaunchKernelGGL(k_recon1d<float>, dim3(n), dim3(phase_threads(S)), 0, st, d_S);
     return hipGetLastError();
 }
 
 hipError_t launch_evolve1d(bool fp64, const Solve1D& S, const Solve1D* d_S, hipStream_t st) {
     const int n = S.face_hi - S.face_lo + 1;
     if (n > 0) {
-        dim3 g(n), b(256);
+        dim3 g(n), b(phase_threads(S));
 #define KBG_E(T, NF, VD, FL) hipLaunchKernelGGL((k_evolve1d<T, NF, VD, FL>), g, b, 0, st, d_S)
 #define KBG_EALL(T)                                                          \
     if (S.vdim == 1 && S.nf == 1 && S.flux == KFVS) KBG_E(T, 1, 1, KFVS);    \
@@ -432,15 +436,15 @@ hipError_t launch_evolve1d(bool fp64, const Solve1D& S, const Solve1D* d_S, hipS
 
 hipError_t launch_wall1d(bool fp64, const Solve1D& S, const Solve1D* d_S, int fused, int par, hipStream_t st) {
     if (!(S.wallL || S.wallR)) return hipSuccess;
-    if (fp64) hipLaunchKernelGGL(k_wall1d<double>, dim3(2), dim3(256), 0, st, d_S, fused, par);
-    else hipLaunchKernelGGL(k_wall1d<float>, dim3(2), dim3(256), 0, st, d_S, fused, par);
+    if (fp64) hipLaunchKernelGGL(k_wall1d<double>, dim3(2), dim3(phase_threads(S)), 0, st, d_S, fused, par);
+    else hipLaunchKernelGGL(k_wall1d<float>, dim3(2), dim3(phase_threads(S)), 0, st, d_S, fused, par);
     return hipGetLastError();
 }
 
 hipError_t launch_update1d(bool fp64, const Solve1D& S, const Solve1D* d_S, double* d_rcell, hipStream_t st) {
     const int n = S.upd_hi - S.upd_lo + 1;
     if (n <= 0) return hipSuccess;
-    dim3 g(n), b(256);
+    dim3 g(n), b(phase_threads(S));
 #define KBG_U(T, NF, VD) hipLaunchKernelGGL((k_update1d<T, NF, VD>), g, b, 0, st, d_S, d_rcell)
     if (fp64) {
         if (S.vdim == 1 && S.nf == 1) KBG_U(double, 1, 1);

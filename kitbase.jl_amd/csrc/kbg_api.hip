@@ -65,6 +65,21 @@ struct kbg_ctx {
     size_t descs2_cap = 0;
     int* d_pref2 = nullptr;
     size_t pref2_cap = 0;
+    // HIP graphs of a batch's steps: [0] a step pair (parities 0, 1), [1] one step (parity 0),
+    // captured for the launch plan whose signature is gsig
+    bool graphs = false;
+    std::string gsig;
+    hipGraph_t graph[2] = {};
+    hipGraphExec_t gexec[2] = {};
+    void drop_graphs() {
+        for (int k = 0; k < 2; k++) {
+            if (gexec[k]) (void)hipGraphExecDestroy(gexec[k]);
+            if (graph[k]) (void)hipGraphDestroy(graph[k]);
+            gexec[k] = nullptr;
+            graph[k] = nullptr;
+        }
+        gsig.clear();
+    }
 };
 
 struct kbg_solve {
@@ -447,8 +462,8 @@ void build_items(kbg_solve* s, int seg, int solve_index, std::vector<WorkItem>& 
     }
 }
 
-int step_phase_1d(kbg_solve* s, double dt, double* residual);
-int step_phase_2d(kbg_solve* s, double dt, double* residual);
+int step_phase_1d(kbg_solve* s, double dt, double* residual, hipStream_t st);
+int step_phase_2d(kbg_solve* s, double dt, double* residual, hipStream_t st);
 
 }  // namespace
 
@@ -494,6 +509,7 @@ int kbg_ctx_destroy(kbg_ctx* c) {
     for (auto& ev : c->join)
         if (ev) (void)hipEventDestroy(ev);
     if (c->fork) (void)hipEventDestroy(c->fork);
+    c->drop_graphs();
     (void)hipFree(c->d_descs);
     (void)hipFree(c->d_items);
     (void)hipFree(c->d_rpart);
@@ -530,6 +546,16 @@ int kbg_ctx_get_schedule(kbg_ctx* c, int* seg_cells, int* streams) {
     if (!c) return fail(KBG_ERR_ARG, "ctx is NULL");
     if (seg_cells) *seg_cells = c->seg_cells;
     if (streams) *streams = c->streams;
+    return KBG_OK;
+}
+
+int kbg_ctx_set_graphs(kbg_ctx* ctx, int enable) {
+    if (!ctx) return fail(KBG_ERR_ARG, "ctx is NULL");
+    ctx->graphs = enable != 0;
+    if (!ctx->graphs) {
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        ctx->drop_graphs();
+    }
     return KBG_OK;
 }
 
@@ -887,12 +913,12 @@ int kbg_update(kbg_solve* s, double dt, double* residual) {
         if ((rc = ensure_phase_buffers(s))) return rc;
         s->h1.dt = dt;
         if ((rc = push1d(s))) return rc;
-        rc = step_phase_1d(s, dt, residual);
+        rc = step_phase_1d(s, dt, residual, s->ctx->stream);
     } else {
         if ((rc = ensure_phase_buffers(s))) return rc;
         s->h2.dt = dt;
         if ((rc = push2d(s))) return rc;
-        rc = step_phase_2d(s, dt, residual);
+        rc = step_phase_2d(s, dt, residual, s->ctx->stream);
     }
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(s->ctx->stream));
@@ -952,6 +978,7 @@ int kbg_batch_step(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double* 
         bool fp64;
         int flux, threads, n, ncells, nfaces, ninterior;
         size_t desc0, cpref0, fpref0, ipref0;
+        double work;
     };
     std::vector<Launch2> launches2;
     std::vector<Solve2D> descs2;
@@ -961,7 +988,7 @@ int kbg_batch_step(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double* 
         for (int i : fused2) groups[{solves[i]->fp64 ? 1 : 0, solves[i]->d.flux, solves[i]->thr2}].push_back(i);
         for (auto& kv : groups) {
             Launch2 L{std::get<0>(kv.first) == 1, std::get<1>(kv.first), std::get<2>(kv.first), (int)kv.second.size(), 0, 0, 0,
-                      descs2.size(), 0, 0, 0};
+                      descs2.size(), 0, 0, 0, 0.0};
             std::vector<int> cp{0}, fp{0}, ip{0};
             for (int i : kv.second) {
                 kbg_solve* s = solves[i];
@@ -973,6 +1000,7 @@ int kbg_batch_step(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double* 
                 cp.push_back(int(cp.back() + nc));
                 fp.push_back(int(fp.back() + nf));
                 ip.push_back(int(ip.back() + ni));
+                L.work += double(nc) * s->d.nv * 2;
             }
             L.ncells = cp.back();
             L.nfaces = fp.back();
@@ -1004,19 +1032,37 @@ int kbg_batch_step(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double* 
         }
     }
     const int seg = std::max(1, ctx->seg_cells);
-    // the fused solves are spread over the streams by work (cells x velocities), heaviest first;
-    // solves are independent, so each stream advances its own share
-    const int nstreams = std::max(1, std::min<int>(ctx->streams > 0 ? ctx->streams : 1 + kbg_ctx::kSide, (int)fused.size()));
-    std::vector<int> stream_of(n, 0);
+    // Independent units -- fused 1D solves, fused 2D groups and solves stepped by the phase kernels
+    // -- are spread over the streams by work (cells x velocities x distributions), heaviest first;
+    // each stream advances its own share.
+    const int units = int(fused.size() + launches2.size() + others.size());
+    const int nstreams = std::max(1, std::min<int>(ctx->streams > 0 ? ctx->streams : 1 + kbg_ctx::kSide, units));
+    std::vector<int> stream_of(n, 0), stream2(launches2.size(), 0);
     {
-        std::vector<int> order(fused);
         auto work = [&](int i) { return double(solves[i]->N) * solves[i]->d.nv * solves[i]->d.nf; };
-        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work(a) > work(b); });
+        std::vector<std::pair<double, int>> order;  // (work, unit): unit >= 0 a solve, < 0 a 2D group (-1 - g)
+        for (int i : fused) order.push_back({work(i), i});
+        for (int i : others) order.push_back({work(i), i});
+        for (size_t g = 0; g < launches2.size(); g++) order.push_back({launches2[g].work, -1 - int(g)});
+        std::stable_sort(order.begin(), order.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
         std::vector<double> load(nstreams, 0.0);
-        for (int i : order) {
+        for (const auto& u : order) {
             const int k = int(std::min_element(load.begin(), load.end()) - load.begin());
-            stream_of[i] = k;
-            load[k] += work(i);
+            if (u.second >= 0) stream_of[u.second] = k;
+            else stream2[-1 - u.second] = k;
+            load[k] += u.first;
+        }
+    }
+    // solves stepped by the phase kernels: buffers and descriptors before the streams fork
+    for (int i : others) {
+        kbg_solve* s = solves[i];
+        if ((rc = ensure_phase_buffers(s))) return rc;
+        if (s->d.dim == 1) {
+            s->h1.dt = dts[i];
+            if ((rc = push1d(s))) return rc;
+        } else {
+            s->h2.dt = dts[i];
+            if ((rc = push2d(s))) return rc;
         }
     }
     // descriptors, work items and packs; one launch per (stream, kernel instance)
@@ -1089,20 +1135,28 @@ int kbg_batch_step(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double* 
         HIPCHK(hipMemcpyAsync(ctx->d_packs, packs.data(), packs.size() * sizeof(kbg::Pack), hipMemcpyHostToDevice, st));
         HIPCHK(hipStreamSynchronize(st));
     }
-    HIPCHK(hipEventRecord(ctx->ev[KBG_EV_STEPS_BEGIN], st));
     // stream k > 0 is side stream k - 1; they fork from and join back into the context stream
     const int nside = nstreams - 1;
     std::vector<hipStream_t> sst(nstreams, st);
-    if (nside > 0) {
-        HIPCHK(hipEventRecord(ctx->fork, st));
-        for (int k = 0; k < nside; k++) {
-            HIPCHK(hipStreamWaitEvent(ctx->side[k], ctx->fork, 0));
-            sst[k + 1] = ctx->side[k];
+    for (int k = 0; k < nside; k++) sst[k + 1] = ctx->side[k];
+    auto fork = [&]() -> int {
+        if (nside > 0) {
+            HIPCHK(hipEventRecord(ctx->fork, st));
+            for (int k = 0; k < nside; k++) HIPCHK(hipStreamWaitEvent(ctx->side[k], ctx->fork, 0));
         }
-    }
-    for (int t = 0; t < nsteps; t++) {
-        const int par = t & 1;
-        const bool last = t == nsteps - 1;
+        return KBG_OK;
+    };
+    auto join = [&]() -> int {
+        for (int k = 0; k < nside; k++) {
+            HIPCHK(hipEventRecord(ctx->join[k], ctx->side[k]));
+            HIPCHK(hipStreamWaitEvent(st, ctx->join[k], 0));
+        }
+        return KBG_OK;
+    };
+    // one time step of every unit on its stream: par is the step parity (which f / w buffer the
+    // fused 1D sweep reads), last marks the step whose residual rows are kept
+    auto enqueue_step = [&](int par, bool last) -> int {
+        int e;
         for (const Launch& L : launches) {
             for (int di : L.members) {
                 const Solve1D& S = descs[desc_of[di]];
@@ -1112,39 +1166,112 @@ int kbg_batch_step(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double* 
             HIPCHK(kbg::launch_sweep1d(L.fp64, L.nf, L.flux, L.wg_waves, ctx->d_descs, ctx->d_items, ctx->d_packs + L.pack0,
                                        (int)L.npacks, par, (last && residuals) ? ctx->d_rpart : nullptr, sst[L.stream]));
         }
-        for (const Launch2& L : launches2) {
+        for (size_t g = 0; g < launches2.size(); g++) {
+            const Launch2& L = launches2[g];
             const Solve2D* dd = ctx->d_descs2 + L.desc0;
-            HIPCHK(kbg::launch_face2d(true, L.flux, dd, ctx->d_pref2 + L.ipref0, L.n, L.ninterior, st));
-            HIPCHK(kbg::launch_flux2d(L.fp64, L.flux, L.threads, dd, ctx->d_pref2 + L.fpref0, L.n, L.nfaces, st));
-            HIPCHK(kbg::launch_cell2d(L.fp64, L.threads, dd, ctx->d_pref2 + L.cpref0, L.n, L.ncells, (last && residuals) ? 1 : 0, st));
+            hipStream_t ss = sst[stream2[g]];
+            HIPCHK(kbg::launch_face2d(true, L.flux, dd, ctx->d_pref2 + L.ipref0, L.n, L.ninterior, ss));
+            HIPCHK(kbg::launch_flux2d(L.fp64, L.flux, L.threads, dd, ctx->d_pref2 + L.fpref0, L.n, L.nfaces, ss));
+            HIPCHK(kbg::launch_cell2d(L.fp64, L.threads, dd, ctx->d_pref2 + L.cpref0, L.n, L.ncells, (last && residuals) ? 1 : 0, ss));
         }
         for (int i : others) {
             kbg_solve* s = solves[i];
+            hipStream_t ss = sst[stream_of[i]];
             double* r = (last && residuals) ? residuals + 5 * (size_t)i : nullptr;
             if (s->d.dim == 1) {
-                if ((rc = ensure_phase_buffers(s))) return rc;
-                if (t == 0) {
-                    s->h1.dt = dts[i];
-                    if ((rc = push1d(s))) return rc;
-                }
-                HIPCHK(kbg::launch_recon1d(s->fp64, s->h1, s->d1, st));
-                HIPCHK(kbg::launch_evolve1d(s->fp64, s->h1, s->d1, st));
-                HIPCHK(kbg::launch_wall1d(s->fp64, s->h1, s->d1, 0, 0, st));
-                if ((rc = step_phase_1d(s, dts[i], r))) return rc;
+                HIPCHK(kbg::launch_recon1d(s->fp64, s->h1, s->d1, ss));
+                HIPCHK(kbg::launch_evolve1d(s->fp64, s->h1, s->d1, ss));
+                HIPCHK(kbg::launch_wall1d(s->fp64, s->h1, s->d1, 0, 0, ss));
+                if ((e = step_phase_1d(s, dts[i], r, ss))) return e;
             } else {
-                if (t == 0) {
-                    s->h2.dt = dts[i];
-                    if ((rc = push2d(s))) return rc;
-                }
-                HIPCHK(kbg::launch_recon2d(s->fp64, s->h2, s->d2, st));
-                HIPCHK(kbg::launch_evolve2d(s->fp64, s->h2, s->d2, st));
-                if ((rc = step_phase_2d(s, dts[i], r))) return rc;
+                HIPCHK(kbg::launch_recon2d(s->fp64, s->h2, s->d2, ss));
+                HIPCHK(kbg::launch_evolve2d(s->fp64, s->h2, s->d2, ss));
+                if ((e = step_phase_2d(s, dts[i], r, ss))) return e;
             }
         }
+        return KBG_OK;
+    };
+    // Without residuals the steps are replayed from HIP graphs: a step pair and a single step,
+    // captured once per launch plan (the signature below: every launch's shape, stream and
+    // arguments; descriptors are read from device memory, re-uploaded by every call) and reused
+    // by later calls on the same batch.
+    const bool use_graph = ctx->graphs && !residuals && nsteps >= 2;
+    std::string sig;
+    if (use_graph) {
+        auto put = [&](const void* p, size_t b) { sig.append(static_cast<const char*>(p), b); };
+        auto puti = [&](long long v) { put(&v, sizeof v); };
+        puti(nstreams);
+        const void* bufs[] = {ctx->d_descs, ctx->d_items, ctx->d_packs, ctx->d_descs2, ctx->d_pref2};
+        put(bufs, sizeof bufs);
+        for (Solve1D d : descs) {
+            d.cur = 0;  // the parity base flips between calls; the kernels read it from the descriptor
+            put(&d, sizeof d);
+        }
+        put(items.data(), items.size() * sizeof(WorkItem));
+        put(packs.data(), packs.size() * sizeof(kbg::Pack));
+        for (const Launch& L : launches) {
+            for (long long v : {(long long)L.fp64, (long long)L.nf, (long long)L.flux, (long long)L.wg_waves, (long long)L.stream,
+                                (long long)L.pack0, (long long)L.npacks})
+                puti(v);
+            for (int di : L.members) {
+                puti(di);
+                puti(desc_of[di]);
+                puti(solves[di]->fp64);
+            }
+            puti(-1);
+        }
+        put(descs2.data(), descs2.size() * sizeof(Solve2D));
+        put(pref2.data(), pref2.size() * sizeof(int));
+        for (size_t g = 0; g < launches2.size(); g++) {
+            const Launch2& L = launches2[g];
+            for (long long v : {(long long)L.fp64, (long long)L.flux, (long long)L.threads, (long long)L.n, (long long)L.ncells,
+                                (long long)L.nfaces, (long long)L.ninterior, (long long)L.desc0, (long long)L.cpref0,
+                                (long long)L.fpref0, (long long)L.ipref0, (long long)stream2[g]})
+                puti(v);
+        }
+        for (int i : others) {
+            const kbg_solve* s = solves[i];
+            const void* ids[] = {s, s->d1, s->d2};
+            put(ids, sizeof ids);
+            puti(stream_of[i]);
+            if (s->d.dim == 1) {
+                Solve1D h = s->h1;
+                h.cur = 0;
+                put(&h, sizeof h);
+            } else {
+                put(&s->h2, sizeof s->h2);
+            }
+        }
+        if (sig != ctx->gsig || !ctx->gexec[0] || !ctx->gexec[1]) {
+            ctx->drop_graphs();
+            for (int which = 0; which < 2; which++) {
+                HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+                int e = fork();
+                if (!e) e = enqueue_step(0, false);
+                if (!e && which == 0) e = enqueue_step(1, false);
+                if (!e) e = join();
+                hipGraph_t g = nullptr;
+                const hipError_t ce = hipStreamEndCapture(st, &g);
+                if (e || ce != hipSuccess) {
+                    if (g) (void)hipGraphDestroy(g);
+                    ctx->drop_graphs();
+                    return e ? e : fail(KBG_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+                }
+                ctx->graph[which] = g;
+                HIPCHK(hipGraphInstantiate(&ctx->gexec[which], g, nullptr, nullptr, 0));
+            }
+            ctx->gsig = std::move(sig);
+        }
     }
-    for (int k = 0; k < nside; k++) {
-        HIPCHK(hipEventRecord(ctx->join[k], ctx->side[k]));
-        HIPCHK(hipStreamWaitEvent(st, ctx->join[k], 0));
+    HIPCHK(hipEventRecord(ctx->ev[KBG_EV_STEPS_BEGIN], st));
+    if (use_graph) {
+        for (int t = 0; t + 1 < nsteps; t += 2) HIPCHK(hipGraphLaunch(ctx->gexec[0], st));
+        if (nsteps & 1) HIPCHK(hipGraphLaunch(ctx->gexec[1], st));
+    } else {
+        if ((rc = fork())) return rc;
+        for (int t = 0; t < nsteps; t++)
+            if ((rc = enqueue_step(t & 1, t == nsteps - 1))) return rc;
+        if ((rc = join())) return rc;
     }
     HIPCHK(hipEventRecord(ctx->ev[KBG_EV_STEPS_END], st));
     // residuals of the fused solves: per-item partial rows reduced per solve
@@ -1190,9 +1317,8 @@ int kbg_batch_step(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double* 
 namespace {
 
 // update! on the phase-exact path (kernels k_update1d / k_resid), optional residual download
-int step_phase_1d(kbg_solve* s, double dt, double* residual) {
+int step_phase_1d(kbg_solve* s, double dt, double* residual, hipStream_t st) {
     (void)dt;
-    hipStream_t st = s->ctx->stream;
     double* rc_ = residual ? s->rcell : nullptr;
     HIPCHK(kbg::launch_update1d(s->fp64, s->h1, s->d1, rc_, st));
     if (residual) {
@@ -1204,9 +1330,8 @@ int step_phase_1d(kbg_solve* s, double dt, double* residual) {
     return KBG_OK;
 }
 
-int step_phase_2d(kbg_solve* s, double dt, double* residual) {
+int step_phase_2d(kbg_solve* s, double dt, double* residual, hipStream_t st) {
     (void)dt;
-    hipStream_t st = s->ctx->stream;
     double* rc_ = residual ? s->rcell : nullptr;
     HIPCHK(kbg::launch_update2d(s->fp64, s->h2, s->d2, rc_, st));
     if (residual) {

@@ -395,6 +395,26 @@ KBG_D double bcast(double x, int l) {
     return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 
+// all-reduce of N doubles across a workgroup of up to 16 wavefronts, wavefront totals added in
+// wavefront order; scratch must hold 16*N doubles and is reused by the caller only after the
+// trailing barrier.
+template <int N> KBG_D void block_allreduce(double* v, double* scratch) {
+    wave_allreduce<N>(v);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+    if (lane == 0) {
+#pragma unroll
+        for (int m = 0; m < N; m++) scratch[wv * N + m] = v[m];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < N; m++) {
+        double t = scratch[m];
+        for (int w = 1; w < nwv; w++) t += scratch[w * N + m];
+        v[m] = t;
+    }
+    __syncthreads();
+}
+
 // all-reduce of N doubles across a 256-thread block; scratch must hold 4*N doubles and is reused
 // by the caller only after the trailing barrier.
 template <int N> KBG_D void block_allreduce256(double* v, double* scratch) {

@@ -416,6 +416,10 @@ class Context:
     def set_schedule(self, seg_cells: int, waves: int = 0):
         _lib.check(self.lib.kbg_ctx_set_schedule(self.handle, seg_cells, waves))
 
+    def set_graphs(self, enable: bool):
+        """Replay batch steps through HIP graphs (kbg_ctx_set_graphs; default on)."""
+        _lib.check(self.lib.kbg_ctx_set_graphs(self.handle, 1 if enable else 0))
+
     def schedule(self) -> tuple[int, int]:
         a, b = C.c_int(), C.c_int()
         _lib.check(self.lib.kbg_ctx_get_schedule(self.handle, C.byref(a), C.byref(b)))
