@@ -375,8 +375,13 @@ template <int V, class T> KBG_D T wave_reduce_scatter(T* v) {
         const bool up = (lane & m) != 0;
 #pragma unroll
         for (int i = 0; i < h; i++) {
-            const T send = up ? v[i] : v[i + h];
-            const T keep = up ? v[i + h] : v[i];
+            // the empty register constraints keep the two values in registers: without them the
+            // compiler turns the lane-dependent select of two array elements into a load through a
+            // selected address, which moves the whole array to scratch memory
+            T a = v[i], b = v[i + h];
+            asm volatile("" : "+v"(a), "+v"(b));
+            const T send = up ? a : b;
+            const T keep = up ? b : a;
             v[i] = keep + __shfl_xor(send, m, 64);
         }
     }

@@ -262,12 +262,11 @@ __global__ __launch_bounds__(step2d_max_threads<T>()) void k_flux2d(const Solve2
         T A = 0, lam = 0, Ui = 0, Vi = 0, kb = 0, M0 = 0, M1 = 0, M2 = 0, M3 = 0, M4 = 0;
         T aL[4] = {0, 0, 0, 0}, aR[4] = {0, 0, 0, 0}, aT[4] = {0, 0, 0, 0};
         T c1L = 0, c2L = 0, c1R = 0, c2R = 0, c1T = 0, c2T = 0;
-        double rec0[4] = {0, 0, 0, 0}, rm2 = 0, rm3 = 0, rm4 = 0;
+        const double* rec = (dir == 0 ? S.xcoef : S.ycoef) + (size_t)fi * kFaceCoef;
         if constexpr (FLUX == KFVS) {
             M0 = T(dt * len);
             M1 = T(0.5 * dt * dt * len);
         } else {
-            const double* rec = (dir == 0 ? S.xcoef : S.ycoef) + (size_t)fi * kFaceCoef;
             if constexpr (FLUX == KCU) {
                 const KcuFace<4>& F = *(const KcuFace<4>*)rec;
                 A = T(F.prim[0] * (F.prim[3] / kPi));
@@ -277,8 +276,6 @@ __global__ __launch_bounds__(step2d_max_threads<T>()) void k_flux2d(const Solve2
                 kb = T(K / (2.0 * F.prim[3]));
                 M1 = T(F.Mt1 * len);
                 M2 = T(F.Mt2 * len);
-                for (int m = 0; m < 4; m++) rec0[m] = F.fw0[m];
-                rm2 = F.Mt2;
             } else {
                 const UgksFace2& F = *(const UgksFace2*)rec;
                 A = T(F.prim[0] * (F.prim[3] / kPi));
@@ -297,7 +294,6 @@ __global__ __launch_bounds__(step2d_max_threads<T>()) void k_flux2d(const Solve2
                     aL[m] = T(F.aL[m]);
                     aR[m] = T(F.aR[m]);
                     aT[m] = T(F.aT[m]);
-                    rec0[m] = F.fw0[m];
                 }
                 c1L = T(0.5 * F.aL[3] * kbd);
                 c2L = T(0.5 * F.aL[3] * F.Mxi2);
@@ -305,8 +301,6 @@ __global__ __launch_bounds__(step2d_max_threads<T>()) void k_flux2d(const Solve2
                 c2R = T(0.5 * F.aR[3] * F.Mxi2);
                 c1T = T(0.5 * F.aT[3] * kbd);
                 c2T = T(0.5 * F.aT[3] * F.Mxi2);
-                rm3 = F.Mt[3];
-                rm4 = F.Mt[4];
             }
         }
         T sm[8];
@@ -387,31 +381,38 @@ __global__ __launch_bounds__(step2d_max_threads<T>()) void k_flux2d(const Solve2
         if ((lane & 7) == 0) red[wave * 16 + (lane >> 3)] = t;
         __syncthreads();
         if (tid == 0) {
+            // face totals in wavefront order, then fw in the face frame and back to the global one
             double r[8];
+#pragma unroll
             for (int m = 0; m < 8; m++) {
                 double q = red[m];
                 for (int w = 1; w < NWV; w++) q += red[w * 16 + m];
                 r[m] = q;
             }
-            double l[4], gfw[4];
+            double l0, l1, l2, l3;
             if constexpr (FLUX == KFVS) {
-                l[0] = r[0];
-                l[1] = r[1];
-                l[2] = r[2];
-                l[3] = 0.5 * (r[3] + r[4]);
+                l0 = r[0];
+                l1 = r[1];
+                l2 = r[2];
+                l3 = 0.5 * (r[3] + r[4]);
             } else if constexpr (FLUX == KCU) {
-                l[0] = rec0[0] * len + rm2 * r[0] * len;
-                l[1] = rec0[1] * len + rm2 * r[1] * len;
-                l[2] = rec0[2] * len + rm2 * r[2] * len;
-                l[3] = rec0[3] * len + rm2 * 0.5 * (r[3] + r[4]) * len;
+                const KcuFace<4>& F = *(const KcuFace<4>*)rec;
+                l0 = F.fw0[0] * len + F.Mt2 * r[0] * len;
+                l1 = F.fw0[1] * len + F.Mt2 * r[1] * len;
+                l2 = F.fw0[2] * len + F.Mt2 * r[2] * len;
+                l3 = F.fw0[3] * len + F.Mt2 * 0.5 * (r[3] + r[4]) * len;
             } else {
-                l[0] = (rec0[0] + (rm3 * r[0] - rm4 * r[1])) * len;
-                l[1] = (rec0[1] + (rm3 * r[2] - rm4 * r[3])) * len;
-                l[2] = (rec0[2] + (rm3 * r[4] - rm4 * r[5])) * len;
-                l[3] = (rec0[3] + (rm3 * 0.5 * r[6] - rm4 * 0.5 * r[7])) * len;
+                const UgksFace2& F = *(const UgksFace2*)rec;
+                l0 = (F.fw0[0] + (F.Mt[3] * r[0] - F.Mt[4] * r[1])) * len;
+                l1 = (F.fw0[1] + (F.Mt[3] * r[2] - F.Mt[4] * r[3])) * len;
+                l2 = (F.fw0[2] + (F.Mt[3] * r[4] - F.Mt[4] * r[5])) * len;
+                l3 = (F.fw0[3] + (F.Mt[3] * 0.5 * r[6] - F.Mt[4] * 0.5 * r[7])) * len;
             }
-            global_frame4(l, n1, n2, gfw);
-            for (int m = 0; m < 4; m++) fwout[m] = gfw[m];
+            // global_frame4 (geo_general.jl:6-26)
+            fwout[0] = l0;
+            fwout[1] = l1 * n1 - l2 * n2;
+            fwout[2] = l1 * n2 + l2 * n1;
+            fwout[3] = l3;
         }
         return;
     }
@@ -462,6 +463,7 @@ __global__ __launch_bounds__(step2d_max_threads<T>()) void k_flux2d(const Solve2
     if ((lane & 3) == 0 && (lane >> 2) < 9) red[wave * 16 + (lane >> 2)] = t;
     __syncthreads();
     double r[9];
+#pragma unroll
     for (int m = 0; m < 9; m++) {
         double q = red[m];
         for (int w = 1; w < NWV; w++) q += red[w * 16 + m];
@@ -486,9 +488,11 @@ __global__ __launch_bounds__(step2d_max_threads<T>()) void k_flux2d(const Solve2
     if (tid == 0) {
         const double m0 = r[0] + Aw * r[5], m1 = r[1] + Aw * r[6], m2 = r[2] + Aw * r[7], m3 = r[3] + Aw * r[8];
         const double m4 = r[4] + Aw * kbw * r[5];
-        double l[4] = {m0 * len * dt, m1 * len * dt, m2 * len * dt, (0.5 * m3 + 0.5 * m4) * len * dt}, gfw[4];
-        global_frame4(l, n1, n2, gfw);
-        for (int m = 0; m < 4; m++) fwout[m] = gfw[m];
+        const double l1 = m1 * len * dt, l2 = m2 * len * dt;
+        fwout[0] = m0 * len * dt;
+        fwout[1] = l1 * n1 - l2 * n2;
+        fwout[2] = l1 * n2 + l2 * n1;
+        fwout[3] = (0.5 * m3 + 0.5 * m4) * len * dt;
     }
 }
 
