@@ -588,7 +588,10 @@ template <class T, int NF, int FLUX> constexpr size_t sweep_lds_item(int nw) {
 
 // occupancy target: 4 wavefronts per SIMD (<= 128 VGPRs) for the FP32 one-distribution sweeps,
 // whatever the register file allows for the others
-template <class T, int NF, int FLUX> constexpr int sweep_waves_per_eu() { return sizeof(T) == 4 && NF == 1 ? 4 : 1; }
+#ifndef KBG_SWEEP_WPE_F32_1F
+#define KBG_SWEEP_WPE_F32_1F 4
+#endif
+template <class T, int NF, int FLUX> constexpr int sweep_waves_per_eu() { return sizeof(T) == 4 && NF == 1 ? KBG_SWEEP_WPE_F32_1F : 1; }
 
 // One workgroup per pack; packs of one XCD (workgroup ids congruent mod 8) are a contiguous run,
 // so neighbouring segments share that XCD's L2.  A pack's items sit side by side: wavefront w
