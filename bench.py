@@ -124,6 +124,8 @@ def main():
     ap.add_argument("--serve", type=int, default=48, metavar="N",
                     help="serving leg: N requests of the mixed stream (0 skips it)")
     ap.add_argument("--serve-rate", type=float, default=100.0, help="Poisson arrival rate of the serving leg (1/s)")
+    ap.add_argument("--serve-chunk", type=int, default=100, help="steps per continuous-batching chunk of the serving leg")
+    ap.add_argument("--serve-serial", action="store_true", help="serving leg: prepare arrivals between chunks only")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 5 times (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     a = ap.parse_args()
@@ -255,9 +257,11 @@ def main():
     if a.serve:
         reqs = [kb.serving.Request(rid=k, cfg=cfg, nsteps=ns, arrival=t)
                 for k, (kind, cfg, ns, t) in enumerate(kb.configs.serving_requests(a.serve, a.serve_rate))]
-        srv = kb.serving.Server(kb.serving.DeviceBackend(kb, ctx=ctx), budget_bytes=32 << 30, chunk_steps=25)
+        srv = kb.serving.Server(kb.serving.DeviceBackend(kb, ctx=ctx), budget_bytes=32 << 30, chunk_steps=a.serve_chunk,
+                                overlap=not a.serve_serial)
         st = srv.run(reqs)
-        serve = dict(st.summary(), rate_per_s=a.serve_rate, budget_bytes=32 << 30, chunk_steps=25,
+        serve = dict(st.summary(), rate_per_s=a.serve_rate, budget_bytes=32 << 30, chunk_steps=a.serve_chunk,
+                     overlap=not a.serve_serial,
                      mix="75% 1D tubes (to t=0.2), 15% 2D cavities 32^2-48^2 x 16^2-24^2 (100 steps), "
                          "10% 1D1F3V 24 cells x 24^3 (40 steps); FP32")
         log(f"[bench] serving: {serve['requests']} requests at {a.serve_rate}/s, p50 {serve['p50_ms']:.1f} ms, "

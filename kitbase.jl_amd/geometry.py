@@ -76,18 +76,19 @@ def pspace2d(X0, X1, NX, Y0, Y1, NY) -> PSpace2D:
     vt[..., 3, 1] = y + 0.5 * dy
     areas = np.empty(x.shape + (4,))
     n = np.empty(x.shape + (4, 2))
-    for j in range(NY):
-        for i in range(NX):
-            c = np.array([x[j, i], y[j, i]])
-            for k in range(4):
-                p1, p2 = vt[j, i, k], vt[j, i, (k + 1) % 4]
-                dd = p1 - p2
-                areas[j, i, k] = np.sqrt(dd[0] * dd[0] + dd[1] * dd[1])
-                nk = _unit_normal(p1, p2)
-                mid = c - (p1 + p2) / 2
-                if not (nk[0] * mid[0] + nk[1] * mid[1] <= 0):
-                    nk = -nk
-                n[j, i, k] = nk
+    # edge lengths and outward unit normals of every cell (geo_struct.jl:98-175), elementwise over
+    # the mesh: the same operations per edge as unit_normal (_unit_normal) and the orientation test
+    for k in range(4):
+        p1, p2 = vt[..., k, :], vt[..., (k + 1) % 4, :]
+        e0, e1 = p1[..., 0] - p2[..., 0], p1[..., 1] - p2[..., 1]
+        areas[..., k] = np.sqrt(e0 * e0 + e1 * e1)
+        d0, d1 = p2[..., 0] - p1[..., 0], p2[..., 1] - p1[..., 1]
+        length = np.sqrt(d0 * d0 + d1 * d1) + 1e-6
+        nk0, nk1 = -d1 / length, d0 / length
+        mid0, mid1 = x - (p1[..., 0] + p2[..., 0]) / 2, y - (p1[..., 1] + p2[..., 1]) / 2
+        flip = ~(nk0 * mid0 + nk1 * mid1 <= 0)
+        n[..., k, 0] = np.where(flip, -nk0, nk0)
+        n[..., k, 1] = np.where(flip, -nk1, nk1)
     return PSpace2D(X0, X1, NX, Y0, Y1, NY, x, y, dx, dy, vt, areas, n)
 
 

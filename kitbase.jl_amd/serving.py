@@ -16,7 +16,8 @@ The loop talks to the device through a backend object (DeviceBackend here; tests
 """
 from __future__ import annotations
 
-import copy
+import dataclasses
+import threading
 import time
 from dataclasses import dataclass, field
 from typing import Any
@@ -75,7 +76,10 @@ class DeviceBackend:
         self.kb.batch_step(handles, dts, nsteps, ctx=self.ctx)
 
     def finish(self, handle, prepared):
-        ctr = copy.deepcopy(prepared[1])
+        # fresh cell arrays for the result (the slope arrays of the prepared state stay zero and
+        # are shared: a download does not touch them)
+        p = prepared[1]
+        ctr = dataclasses.replace(p, w=np.empty_like(p.w), prim=np.empty_like(p.prim), f=np.empty_like(p.f))
         handle.download(ctr)
         handle.close()
         return ctr
@@ -96,6 +100,8 @@ class ServeStats:
     latencies_ms: list = field(default_factory=list)
     updates: float = 0.0
     wall_s: float = 0.0
+    step_s: float = 0.0       # time inside backend.step (device work plus launch overhead)
+    host_s: float = 0.0       # time preparing, uploading and downloading requests
     chunks: int = 0
     max_live: int = 0
     peak_bytes: int = 0
@@ -109,13 +115,20 @@ class ServeStats:
             "max_ms": float(lat.max()) if lat.size else None,
             "updates_per_s": self.updates / self.wall_s if self.wall_s > 0 else 0.0,
             "chunks": self.chunks,
+            "step_s": self.step_s,
+            "host_s": self.host_s,
             "max_live": self.max_live,
             "peak_bytes": self.peak_bytes,
         }
 
 
 class Server:
-    def __init__(self, backend, budget_bytes: int, chunk_steps: int = 25, clock=time.perf_counter, sleep=time.sleep):
+    """overlap: while a chunk runs on the device (backend.step on a worker thread; the library call
+    releases the interpreter lock), the calling thread prepares the requests that arrive meanwhile
+    (host-side set-up only: no device calls), so host set-up does not stall the live requests."""
+
+    def __init__(self, backend, budget_bytes: int, chunk_steps: int = 25, clock=time.perf_counter, sleep=time.sleep,
+                 overlap: bool = False):
         if chunk_steps < 1:
             raise ValueError("chunk_steps must be >= 1")
         self.backend = backend
@@ -123,6 +136,7 @@ class Server:
         self.chunk = int(chunk_steps)
         self.clock = clock
         self.sleep = sleep
+        self.overlap = overlap
 
     def run(self, requests: list[Request]) -> ServeStats:
         stats = ServeStats()
@@ -132,19 +146,27 @@ class Server:
         used = 0
         t0 = self.clock()
         now = lambda: self.clock() - t0
-        while pending or waiting or live:
+
+        def prepare_arrived():
             t = now()
             while pending and pending[0].arrival <= t:
                 r = pending.pop(0)
+                h0 = self.clock()
                 prep = self.backend.prepare(r)
                 nb = self.backend.footprint(prep, r)
+                stats.host_s += self.clock() - h0
                 if nb > self.budget:
                     raise ValueError(f"request {r.rid} needs {nb} bytes, more than the budget {self.budget}")
                 waiting.append((r, prep, nb))
+
+        while pending or waiting or live:
+            prepare_arrived()
             # first come, first served while the footprint fits
             while waiting and used + waiting[0][2] <= self.budget:
                 r, prep, nb = waiting.pop(0)
+                h0 = self.clock()
                 h, dt = self.backend.create(prep, r)
+                stats.host_s += self.clock() - h0
                 r.admitted = now()
                 live.append(_Live(r, h, dt, r.nsteps, nb, prep))
                 used += nb
@@ -155,13 +177,37 @@ class Server:
                     self.sleep(max(0.0, pending[0].arrival - now()))
                 continue
             k = min(self.chunk, min(l.left for l in live))
-            self.backend.step([l.handle for l in live], [l.dt for l in live], k)
+            h0 = self.clock()
+            if self.overlap:
+                err = []
+
+                def run():
+                    try:
+                        self.backend.step([l.handle for l in live], [l.dt for l in live], k)
+                    except BaseException as e:  # re-raised on the calling thread
+                        err.append(e)
+
+                th = threading.Thread(target=run)
+                th.start()
+                while th.is_alive():
+                    if pending and pending[0].arrival <= now():
+                        prepare_arrived()
+                    else:
+                        th.join(timeout=5e-4)
+                th.join()
+                if err:
+                    raise err[0]
+            else:
+                self.backend.step([l.handle for l in live], [l.dt for l in live], k)
+            stats.step_s += self.clock() - h0
             stats.chunks += 1
             for l in live:
                 l.left -= k
                 stats.updates += float(k) * l.req.points
             for l in [l for l in live if l.left == 0]:
+                h0 = self.clock()
                 l.req.result = self.backend.finish(l.handle, l.prepared)
+                stats.host_s += self.clock() - h0
                 l.req.done = now()
                 stats.latencies_ms.append(1e3 * l.req.latency)
                 used -= l.nbytes

@@ -340,9 +340,15 @@ def init_fvm(ks: SolverSet) -> tuple[ControlVolumes, Interfaces]:
     nx, ny = ps2.nx, ps2.ny
     w = np.empty((ny, nx, 4))
     f = np.empty((ny, nx, 2, nv))
+    seen2: dict[bytes, tuple[int, int]] = {}  # piecewise-constant initial states: evaluate each once
     for j in range(ny):
         for i in range(nx):
             w[j, i] = ks.ib.fw(ps2.x[j, i], ps2.y[j, i])
+            key = w[j, i].tobytes()
+            if key in seen2:
+                f[j, i] = f[seen2[key]]
+                continue
+            seen2[key] = (j, i)
             f[j, i, 0], f[j, i, 1] = ks.ib.ff(ps2.x[j, i], ps2.y[j, i])
     prim = conserve_prim(w, g)
     ctr = ControlVolumes(w, prim, f, np.zeros((ny, nx, 2, 4)), np.zeros((ny, nx, 2, 2, nv)))

@@ -146,3 +146,41 @@ def test_autotuner_picks_the_fastest_and_caches(tmp_path):
         assert best2 == (60, 2) and taken2 == 0 and ctx.sched == (60, 2)
     finally:
         kb.solver.batch_step = orig
+
+
+def test_overlap_prepares_arrivals_while_a_chunk_runs():
+    """overlap=True: requests that arrive while a chunk runs are prepared during it (host work
+    beside the device call), never admitted into the running chunk, and every request still gets
+    exactly its steps."""
+    import threading
+    import time
+
+    class SlowBackend(FakeBackend):
+        def __init__(self):
+            super().__init__(clock=None)
+            self.prepared_during_step = 0
+            self.stepping = threading.Event()
+
+        def prepare(self, r):
+            if self.stepping.is_set():
+                self.prepared_during_step += 1
+            return super().prepare(r)
+
+        def step(self, handles, dts, nsteps):
+            assert set(handles) <= self.live
+            self.calls.append((tuple(handles), nsteps))
+            self.stepping.set()
+            time.sleep(0.02)
+            self.stepping.clear()
+
+    be = SlowBackend()
+    reqs = [R(0, 30, 0.0), R(1, 20, 0.005), R(2, 10, 0.012), R(3, 10, 0.03)]
+    srv = serving.Server(be, 1000, chunk_steps=10, overlap=True)
+    st = srv.run(reqs)
+    done = {}
+    for handles, n in be.calls:
+        for h in handles:
+            done[h] = done.get(h, 0) + n
+    assert done == {0: 30, 1: 20, 2: 10, 3: 10}
+    assert be.prepared_during_step >= 1
+    assert st.summary()["requests"] == 4 and not be.live
