@@ -1,7 +1,7 @@
 """GPU parity: the HIP path (through the C ABI) against the FP64 oracle on the same inputs.
 
 Tolerances: FP64 storage <= 1e-6 relative (observed ~1e-12); FP32 storage <= 1e-4 relative L2 in
-density, velocity and temperature (the `north_star` bar).
+density, velocity and temperature.
 """
 import copy
 
