@@ -116,6 +116,8 @@ def main():
     ap.add_argument("--instances", type=int, default=512)
     ap.add_argument("--seg", type=int, default=0, help="cells per work item (0: chosen by the autotuner)")
     ap.add_argument("--no-tune", action="store_true", help="skip the schedule autotuner (library defaults)")
+    ap.add_argument("--sweep-steps", type=int, default=2, choices=(1, 2),
+                    help="time steps per fused sweep launch (2: two per pass over f where the solves allow it)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 single-solve comparison")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU work of the baseline sample")
@@ -169,6 +171,7 @@ def main():
         f"(host setup {time.time() - t0:.1f}s)")
 
     ctx = kb.Context(0)
+    ctx.set_sweep_steps(a.sweep_steps)
     dts = np.array([d for *_, d in inst])
 
     # ---- FP32 batched path

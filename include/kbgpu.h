@@ -93,6 +93,12 @@ int kbg_ctx_get_schedule(kbg_ctx *ctx, int *seg_cells, int *streams);
  * launches every kernel one by one.  Same results either way; graphs cut host launch work when the
  * host, not the device, bounds the step rate.  No counterpart in KitBase. */
 int kbg_ctx_set_graphs(kbg_ctx *ctx, int enable);
+/* Time steps per launch of the fused 1D sweep (default 2): with 2, batches whose fused 1D solves all
+ * use KFVS without boundary-face buffers (no Maxwell walls, no stepped cell at a ghost-less
+ * boundary) advance two steps per pass over f -- a 4-cell halo per 56-cell segment, one read and
+ * one write of f per two steps, bit-identical results; 1 always launches one step at a time.
+ * No counterpart in KitBase. */
+int kbg_ctx_set_sweep_steps(kbg_ctx *ctx, int steps);
 /* HIP events on the context stream, for timing: record slot (0..15), elapsed ms between slots.
  * kbg_step / kbg_batch_step record slots KBG_EV_STEPS_BEGIN and KBG_EV_STEPS_END around their
  * time-step loop (after the descriptor upload, before the residual download), so the elapsed

@@ -65,6 +65,7 @@ struct kbg_ctx {
     size_t descs2_cap = 0;
     int* d_pref2 = nullptr;
     size_t pref2_cap = 0;
+    int sweep_steps = 2;  // time steps per fused 1D sweep launch where the solves allow two
     // HIP graphs of a batch's steps: [0] a step pair (parities 0, 1), [1] one step (parity 0),
     // captured for the launch plan whose signature is gsig
     bool graphs = false;
@@ -559,6 +560,13 @@ int kbg_ctx_set_graphs(kbg_ctx* ctx, int enable) {
     return KBG_OK;
 }
 
+int kbg_ctx_set_sweep_steps(kbg_ctx* ctx, int steps) {
+    if (!ctx) return fail(KBG_ERR_ARG, "ctx is NULL");
+    if (steps != 1 && steps != 2) return fail(KBG_ERR_ARG, "sweep steps must be 1 or 2");
+    ctx->sweep_steps = steps;
+    return KBG_OK;
+}
+
 int kbg_ctx_event_record(kbg_ctx* c, int slot) {
     if (!c || slot < 0 || slot >= 16) return fail(KBG_ERR_ARG, "bad event slot");
     HIPCHK(hipEventRecord(c->ev[slot], c->stream));
@@ -1031,7 +1039,15 @@ int kbg_batch_step(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double* 
             HIPCHK(hipStreamSynchronize(st));
         }
     }
-    const int seg = std::max(1, ctx->seg_cells);
+    // two time steps per sweep launch when every fused 1D solve allows it (KFVS, no boundary-face
+    // buffers: no walls, no stepped cell next to a ghost-less boundary); not under graph replay
+    bool two = ctx->sweep_steps == 2 && nsteps >= 2 && !fused.empty() && !(ctx->graphs && !residuals);
+    for (int i : fused) {
+        const Solve1D& S = solves[i]->h1;
+        if (S.flux != kbg::KFVS || S.wallL || S.wallR || (S.faceL_buf && S.upd_lo == 1) || (S.faceR_buf && S.upd_hi == S.nx))
+            two = false;
+    }
+    const int seg = std::min(std::max(1, ctx->seg_cells), two ? kbg::kSweepOut2 : kbg::kSweepOut);
     // Independent units -- fused 1D solves, fused 2D groups and solves stepped by the phase kernels
     // -- are spread over the streams by work (cells x velocities x distributions), heaviest first;
     // each stream advances its own share.
@@ -1153,9 +1169,10 @@ int kbg_batch_step(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double* 
         }
         return KBG_OK;
     };
-    // one time step of every unit on its stream: par is the step parity (which f / w buffer the
-    // fused 1D sweep reads), last marks the step whose residual rows are kept
-    auto enqueue_step = [&](int par, bool last) -> int {
+    // one round of every unit on its stream: nst time steps (1, or 2 for the fused 1D sweep when
+    // `two` holds; the other units are launched nst times); par is the round parity (which f / w
+    // buffer the fused 1D sweep reads), last marks the round whose residual rows are kept
+    auto enqueue_round = [&](int par, int nst, bool last_round) -> int {
         int e;
         for (const Launch& L : launches) {
             for (int di : L.members) {
@@ -1164,8 +1181,10 @@ int kbg_batch_step(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double* 
                     HIPCHK(kbg::launch_wall1d(solves[di]->fp64, S, ctx->d_descs + desc_of[di], 1, par, sst[L.stream]));
             }
             HIPCHK(kbg::launch_sweep1d(L.fp64, L.nf, L.flux, L.wg_waves, ctx->d_descs, ctx->d_items, ctx->d_packs + L.pack0,
-                                       (int)L.npacks, par, (last && residuals) ? ctx->d_rpart : nullptr, sst[L.stream]));
+                                       (int)L.npacks, par, (last_round && residuals) ? ctx->d_rpart : nullptr, nst, sst[L.stream]));
         }
+        for (int ks = 0; ks < nst; ks++) {
+        const bool last = last_round && ks == nst - 1;
         for (size_t g = 0; g < launches2.size(); g++) {
             const Launch2& L = launches2[g];
             const Solve2D* dd = ctx->d_descs2 + L.desc0;
@@ -1188,6 +1207,7 @@ int kbg_batch_step(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double* 
                 HIPCHK(kbg::launch_evolve2d(s->fp64, s->h2, s->d2, ss));
                 if ((e = step_phase_2d(s, dts[i], r, ss))) return e;
             }
+        }
         }
         return KBG_OK;
     };
@@ -1247,8 +1267,8 @@ int kbg_batch_step(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double* 
             for (int which = 0; which < 2; which++) {
                 HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
                 int e = fork();
-                if (!e) e = enqueue_step(0, false);
-                if (!e && which == 0) e = enqueue_step(1, false);
+                if (!e) e = enqueue_round(0, 1, false);
+                if (!e && which == 0) e = enqueue_round(1, 1, false);
                 if (!e) e = join();
                 hipGraph_t g = nullptr;
                 const hipError_t ce = hipStreamEndCapture(st, &g);
@@ -1264,13 +1284,18 @@ int kbg_batch_step(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double* 
         }
     }
     HIPCHK(hipEventRecord(ctx->ev[KBG_EV_STEPS_BEGIN], st));
+    int rounds = use_graph ? nsteps : 0;  // sweep launches (buffer flips) of the fused 1D solves
     if (use_graph) {
         for (int t = 0; t + 1 < nsteps; t += 2) HIPCHK(hipGraphLaunch(ctx->gexec[0], st));
         if (nsteps & 1) HIPCHK(hipGraphLaunch(ctx->gexec[1], st));
     } else {
         if ((rc = fork())) return rc;
-        for (int t = 0; t < nsteps; t++)
-            if ((rc = enqueue_step(t & 1, t == nsteps - 1))) return rc;
+        for (int t = 0; t < nsteps;) {
+            const int nst = two && nsteps - t >= 2 ? 2 : 1;
+            if ((rc = enqueue_round(rounds & 1, nst, t + nst == nsteps))) return rc;
+            t += nst;
+            rounds++;
+        }
         if ((rc = join())) return rc;
     }
     HIPCHK(hipEventRecord(ctx->ev[KBG_EV_STEPS_END], st));
@@ -1297,7 +1322,7 @@ int kbg_batch_step(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double* 
     for (int i = 0; i < n; i++) {
         if (desc_of[i] < 0) continue;
         kbg_solve* s = solves[i];
-        s->h1.cur = (s->h1.cur + nsteps) & 1;
+        s->h1.cur = (s->h1.cur + rounds) & 1;
         if ((rc = push1d(s))) return rc;
     }
     // fused 2D solves: residuals from the last step's rows

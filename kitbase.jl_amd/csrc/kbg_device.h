@@ -342,9 +342,12 @@ using rsrc_t = __amdgpu_buffer_rsrc_t;
 KBG_D rsrc_t make_rsrc(const void* p, unsigned bytes) { return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)bytes, 0x00020000); }
 KBG_D float bload(float*, rsrc_t r, int voff, int soff) { return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0)); }
 KBG_D double bload(double*, rsrc_t r, int voff, int soff) { return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0)); }
-KBG_D void bstore(float x, rsrc_t r, int voff, int soff) { __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), r, voff, soff, 0); }
-KBG_D void bstore(double x, rsrc_t r, int voff, int soff) {
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, x), r, voff, soff, 0);
+// AUX: the instruction's cache-policy bits (0: default)
+template <int AUX = 0> KBG_D void bstore(float x, rsrc_t r, int voff, int soff) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), r, voff, soff, AUX);
+}
+template <int AUX = 0> KBG_D void bstore(double x, rsrc_t r, int voff, int soff) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, x), r, voff, soff, AUX);
 }
 
 // ---- reductions -------------------------------------------------------------------------------

@@ -35,10 +35,23 @@
 #include "kbg_device.h"
 #include "kbg_types.h"
 
+// alternate the pack order between steps (1) or keep it (0)
+#ifndef KBG_SWEEP_ZIGZAG
+#define KBG_SWEEP_ZIGZAG 1
+#endif
+// cache policy of the f^{n+1} stores (0: default)
+#ifndef KBG_SWEEP_STORE_AUX
+#define KBG_SWEEP_STORE_AUX 0
+#endif
+#ifndef KBG_SWEEP_WPE_F32_1F
+#define KBG_SWEEP_WPE_F32_1F 4
+#endif
+
 namespace kbg {
 
 constexpr int kSweepLanes = 64;  // storage cells held by a wavefront
 constexpr int kSweepOut = 60;    // cells a segment updates (lanes 2 .. 61)
+constexpr int kSweepOut2 = 56;   // cells a two-step segment updates (lanes 4 .. 59)
 
 // DPP whole-wavefront shifts: shr1(x)[L] = x[L - 1], shl1(x)[L] = x[L + 1] (0 beyond the ends)
 KBG_D float shr1(float x) {
@@ -116,9 +129,14 @@ template <class T> struct SFace {
     T A, nl2, Ui, kb, Mxi2, M0, M1, M2, M3, M4, aL0, aL1, aL2, aR0, aR1, aR2, aT0, aT1, aT2;
 };
 
-template <class T, int NF, int FLUX, int LIMK, bool SHAK>
+// NSTEP = 2 advances the segment two time steps in one pass (KFVS, no boundary-face buffers): the
+// window holds 4 halo cells per side, step 1's f^{n+1} stays in the registers that carry G, and only
+// f^{n+2} is stored -- one read and one write of f per two steps.  Cells the first step does not
+// update (boundary or ghost cells) keep f^n, reloaded for the second step.
+template <class T, int NF, int FLUX, int LIMK, bool SHAK, int NSTEP>
 KBG_D void sweep1d(const Solve1D& S, const WorkItem& wi, int par, int lane, int wave, int NW, double* lds,
                    double* rrow) {
+    static_assert(NSTEP == 1 || FLUX == KFVS, "two steps per pass are built for KFVS");
     using C = SweepCfg<T, NF, FLUX>;
     constexpr int VW = C::VW, CH = C::CH, NRAW = C::NRAW, NR = C::NR;
     const int nx = S.nx, ng = S.ng, nv = S.nv, N = nx + 2 * ng;
@@ -136,7 +154,7 @@ KBG_D void sweep1d(const Solve1D& S, const WorkItem& wi, int par, int lane, int 
     const int lk = S.limiter;
 
     // ---- per-lane cell
-    const int k0 = wi.c0 - 1 + ng - 2;  // storage cell of lane 0
+    const int k0 = wi.c0 - 1 + ng - 2 * NSTEP;  // storage cell of lane 0
     const int k = k0 + lane;
     const int kc = k < 0 ? 0 : (k >= N ? N - 1 : k);  // clamped: lanes past the ends load valid memory
     const int i = k - ng + 1;                          // physical index
@@ -152,7 +170,14 @@ KBG_D void sweep1d(const Solve1D& S, const WorkItem& wi, int par, int lane, int 
     // KFVS carries half slopes (one multiply less in the limiter): the half cell width doubles
     constexpr bool HALF = FLUX == KFVS;
     const T dxs = HALF ? T(2.0 * dxh64) : dxh;
-    const bool upd = lane >= 2 && lane < 2 + kSweepOut && i >= wi.c0 && i <= wi.c1;
+    // cells stored by this item (lanes 2..61, or 4..59 for two steps), and, for two steps, the
+    // cells the first step updates (every stepped cell with two neighbours per side in the window)
+    const bool upd = NSTEP == 1 ? (lane >= 2 && lane < 2 + kSweepOut && i >= wi.c0 && i <= wi.c1)
+                                : (lane >= 4 && lane < 4 + kSweepOut2 && i >= wi.c0 && i <= wi.c1);
+    const bool upd_mid = NSTEP == 2 && lane >= 2 && lane < 62 && i >= S.upd_lo && i <= S.upd_hi;
+    // a wavefront whose lanes 2..61 (the ones the second step reads) hold cells the first step leaves
+    // alone -- boundary or ghost cells -- reloads their f^n; lanes 0, 1, 62, 63 feed no stored cell
+    const bool keep_any = NSTEP == 2 && __ballot(!upd_mid && lane >= 2 && lane < 62 && k >= 0 && k < N) != 0ull;
     // the lane's left face comes from the boundary-face buffers (only where a stepped cell uses it)
     const bool bufL = (i == 1 && S.faceL_buf && S.upd_lo == 1) || (i == nx + 1 && S.faceR_buf && S.upd_hi == nx);
     const int bside = i == 1 ? 0 : 1;
@@ -229,7 +254,7 @@ KBG_D void sweep1d(const Solve1D& S, const WorkItem& wi, int par, int lane, int 
 #pragma unroll
     for (int m = 0; m < NR; m++) acc[m] = 0.0;
     const gptr<const T> bff = to_global((const T*)S.bff);
-    const T UoT = T(Uo64);
+    T UoT = T(Uo64);  // old velocity of the Shakhov heat flux (step 1's new one for a second step)
 
     // f rows of chunk c into buffer c & 1 (the next chunk's loads are in flight while a chunk computes)
     T frb[2][NF][CH];
@@ -246,8 +271,8 @@ KBG_D void sweep1d(const Solve1D& S, const WorkItem& wi, int par, int lane, int 
 
     // one chunk of CH velocities; UPM: 0 all take the right state, 1 all the left, 2 mixed;
     // BUF: a lane's left face comes from the boundary-face buffers
-    auto chunk1 = [&](auto cc, auto upm, auto buf) {
-        constexpr int c = cc, UPM = upm;
+    auto chunk1 = [&](auto cc, auto upm, auto buf, auto stc) {
+        constexpr int c = cc, UPM = upm, SS = stc;
         constexpr bool BUF = buf;
         const int vb = vbeg + c * CH;
         VC& ch = vt[vb / CH];
@@ -262,7 +287,9 @@ KBG_D void sweep1d(const Solve1D& S, const WorkItem& wi, int par, int lane, int 
             T fq[NF], sq[NF], fu[NF], su[NF], ffq[NF];
 #pragma unroll
             for (int q = 0; q < NF; q++) {
-                const T f = fr[q][j];
+                T f;
+                if constexpr (SS == 0) f = fr[q][j];
+                else f = G[q][c * CH + j];  // the first step's f^{n+1}, still in registers
                 fq[q] = f;
                 sq[q] = sweep_limit<T, LIMK, HALF>(lk, (f - shr1(f)) * rL, (shl1(f) - f) * rR);
             }
@@ -359,226 +386,281 @@ KBG_D void sweep1d(const Solve1D& S, const WorkItem& wi, int par, int lane, int 
     using I2 = std::integral_constant<int, 2>;
     using BT = std::integral_constant<bool, true>;
     using BF = std::integral_constant<bool, false>;
-    load_chunk(I0{});
-    static_for<VW / CH>([&](auto cc) {
-        constexpr int c = cc;
-        const int vb = vbeg + c * CH;
-        if (vb >= vend) return;
-        if constexpr (c + 1 < VW / CH) load_chunk(std::integral_constant<int, c + 1>{});
-        if (LIMK == LIM_VANLEER && !seg_buf) {
-            // velocities are sorted: a chunk whose velocities agree in sign takes one side throughout
-            const int m = vt[vb / CH].upmask;
-            if (m == 0xff) chunk1(cc, I1{}, BF{});
-            else if (m == 0) chunk1(cc, I0{}, BF{});
-            else chunk1(cc, I2{}, BF{});
-        } else if (seg_buf) {
-            chunk1(cc, I2{}, BT{});
-        } else {
-            chunk1(cc, I2{}, BF{});
+    double wkeep[3] = {0.0, 0.0, 0.0}, pkeep[3] = {1.0, 0.0, 1.0};  // wavefront 0: step 1's w, prim
+    static_for<NSTEP>([&](auto sc) {
+        constexpr int s = sc;
+        constexpr bool LAST = s == NSTEP - 1;
+        using SI = std::integral_constant<int, s>;
+        if constexpr (s > 0) {
+#pragma unroll
+            for (int m = 0; m < NR; m++) acc[m] = 0.0;
         }
-    });
-
-    // ---- wavefront 0 adds the wavefronts' sums (fixed order) and runs the cells' step! chain;
-    // the others take its coefficients from LDS
-    T An, nl2, Un, inv, rTi, kbn, pre = 0, Ao = 0, nlo2 = 0, Uo = 0, lo2 = 0, kbo = 0;
-    T* coef = (T*)(lds + (size_t)NW * NR * kSweepLanes);  // [12][64]
-    if (NW > 1) {
-#pragma unroll
-        for (int m = 0; m < NR; m++) lds[(wave * NR + m) * kSweepLanes + lane] = acc[m];
-        __syncthreads();
-    }
-    if (wave == 0) {
-        double tot[NR];
-        if (NW > 1) {
-#pragma unroll
-            for (int m = 0; m < NR; m++) {
-                double sm = 0.0;
-                for (int w = 0; w < NW; w++) sm += lds[(w * NR + m) * kSweepLanes + lane];
-                tot[m] = sm;
+        if constexpr (s == 0) load_chunk(I0{});
+        static_for<VW / CH>([&](auto cc) {
+            constexpr int c = cc;
+            const int vb = vbeg + c * CH;
+            if (vb >= vend) return;
+            if constexpr (s == 0 && c + 1 < VW / CH) load_chunk(std::integral_constant<int, c + 1>{});
+            if (LIMK == LIM_VANLEER && !seg_buf) {
+                // velocities are sorted: a chunk whose velocities agree in sign takes one side throughout
+                const int m = vt[vb / CH].upmask;
+                if (m == 0xff) chunk1(cc, I1{}, BF{}, SI{});
+                else if (m == 0) chunk1(cc, I0{}, BF{}, SI{});
+                else chunk1(cc, I2{}, BF{}, SI{});
+            } else if (seg_buf) {
+                chunk1(cc, I2{}, BT{}, SI{});
+            } else {
+                chunk1(cc, I2{}, BF{}, SI{});
             }
-        } else {
-#pragma unroll
-            for (int m = 0; m < NR; m++) tot[m] = acc[m];
-        }
-        double wo[3], po[3] = {1.0, 0.0, 1.0};
-#pragma unroll
-        for (int m = 0; m < 3; m++) wo[m] = win[(size_t)kc * 3 + m];
-        if (SHAK) {
-#pragma unroll
-            for (int m = 0; m < 3; m++) po[m] = S.prim[(size_t)kc * 3 + m];
-        }
-
-        // face moments of the lane's left face, and of its right face (the next lane's left)
-        double fwl[3];
-        if constexpr (FLUX == KFVS) {
-            fwl[0] = tot[0];
-            fwl[1] = tot[1];
-            fwl[2] = NF == 2 ? 0.5 * (tot[2] + tot[3]) : 0.5 * tot[2];
-        } else if constexpr (FLUX == KCU) {
-            fwl[0] = fw0[0] + m3 * tot[0];
-            fwl[1] = fw0[1] + m3 * tot[1];
-            fwl[2] = fw0[2] + m3 * 0.5 * (NF == 2 ? tot[2] + tot[3] : tot[2]);
-        } else {
-            fwl[0] = fw0[0] + (m3 * tot[0] - m4 * tot[1]);
-            fwl[1] = fw0[1] + (m3 * tot[2] - m4 * tot[3]);
-            fwl[2] = fw0[2] + (m3 * 0.5 * tot[4] - m4 * 0.5 * tot[5]);
-        }
-        if (bufL) {
-#pragma unroll
-            for (int m = 0; m < 3; m++) fwl[m] = S.bfw[bside * 3 + m];
-        }
-        double wn[3], pn[3];
-#pragma unroll
-        for (int m = 0; m < 3; m++) wn[m] = wo[m] + (fwl[m] - shl1(fwl[m])) * rdx64;
-
-        // the cell's step! coefficients (FP64 chain, lane-parallel)
-        conserve_prim<3>(wn, S.gam, pn);
-        const T rT = relax_ratio<T>(pn[0], pn[2], S.mu, S.om, dt);
-        inv = T(1) / (T(1) + rT);
-        rTi = rT * inv;
-        An = T(pn[0] * sqrt(pn[2] / kPi));
-        nl2 = T(-pn[2] * kLog2e);
-        Un = T(pn[1]);
-        kbn = T(K / (2.0 * pn[2]));
-        if (SHAK && shak) {
-            const double q = 0.5 * tot[NRAW];
-            pre = T(0.8 * (1.0 - S.Pr) * (po[2] * po[2]) / po[0] * q);
-            Ao = T(po[0] * sqrt(po[2] / kPi));
-            nlo2 = T(-po[2] * kLog2e);
-            Uo = T(po[1]);
-            lo2 = T(2.0 * po[2]);
-            kbo = T(K / (2.0 * po[2]));
-        }
-        if (upd) {
-            const size_t r = (size_t)kc * 3;
-            const gptr<double> wout = to_global(S.w[in ^ 1]);
-            const gptr<double> gp = to_global(S.prim);
-#pragma unroll
-            for (int m = 0; m < 3; m++) {
-                wout[r + m] = wn[m];
-                gp[r + m] = pn[m];
-            }
-        }
-        if (rrow) {
-            // residual sums (solver_update.jl:86-88, update_boundary.jl:41-44): interior / boundary cells
-            double r[12];
-            const bool bnd = i == 1 || i == nx;
-#pragma unroll
-            for (int m = 0; m < 3; m++) {
-                const double d2 = upd ? (wn[m] - wo[m]) * (wn[m] - wo[m]) : 0.0, a = upd ? fabs(wn[m]) : 0.0;
-                r[m] = bnd ? 0.0 : d2;
-                r[3 + m] = bnd ? 0.0 : a;
-                r[6 + m] = bnd ? d2 : 0.0;
-                r[9 + m] = bnd ? a : 0.0;
-            }
-            wave_allreduce<12>(r);
-            double y = 0.0;
-#pragma unroll
-            for (int m = 0; m < 12; m++)
-                if (lane == m) y = r[m];
-            if (lane < 12) rrow[lane] = y;
-        }
-        if (NW > 1) {
-            const T cv[12] = {An, nl2, Un, inv, rTi, kbn, pre, Ao, nlo2, Uo, lo2, kbo};
-#pragma unroll
-            for (int m = 0; m < 12; m++) coef[m * kSweepLanes + lane] = cv[m];
-        }
-    }
-    if (NW > 1) {
-        __syncthreads();
-        if (wave != 0) {
-            An = coef[0 * kSweepLanes + lane];
-            nl2 = coef[1 * kSweepLanes + lane];
-            Un = coef[2 * kSweepLanes + lane];
-            inv = coef[3 * kSweepLanes + lane];
-            rTi = coef[4 * kSweepLanes + lane];
-            kbn = coef[5 * kSweepLanes + lane];
-            if (SHAK) {
-                pre = coef[6 * kSweepLanes + lane];
-                Ao = coef[7 * kSweepLanes + lane];
-                nlo2 = coef[8 * kSweepLanes + lane];
-                Uo = coef[9 * kSweepLanes + lane];
-                lo2 = coef[10 * kSweepLanes + lane];
-                kbo = coef[11 * kSweepLanes + lane];
-            }
-        }
-    }
-
-    // ---- pass 2: relaxation toward M (+ Shakhov) and the store of f^{n+1}
-    // M at the chunk's nodes: one exp per node, or -- FP32 on equally spaced nodes -- the product
-    // recurrence M(u + du) = M(u) R, R <- R Q with R = exp(-lam du (2 (u - U) + du)), Q = exp(-2 lam du^2),
-    // restarted every chunk of 8 nodes (at most ~30 roundings against the exp per node)
-    auto maxw = [&](VC& ch, T A, T nlam2, T U0, T (&M)[CH]) {
-        if (sizeof(T) == 4 && ch.du != T(0)) {
-            const T d = ch.du, c0 = ch.u[0] - U0;
-            T m = A * texp2(nlam2 * (c0 * c0));
-            T R = texp2(nlam2 * d * (T(2) * c0 + d));
-            const T Q = texp2(T(2) * nlam2 * (d * d));
-            static_for<CH>([&](auto jc) {
-                constexpr int j = jc;
-                M[j] = m;
-                if constexpr (j + 1 < CH) {
-                    m *= R;
-                    R *= Q;
-                }
-            });
-        } else {
-            static_for<CH>([&](auto jc) {
-                constexpr int j = jc;
-                const T c = ch.u[j] - U0;
-                M[j] = A * texp2(nlam2 * (c * c));
-            });
-        }
-    };
-    static_for<VW / CH>([&](auto cc) {
-        constexpr int c = cc;
-        const int vb = vbeg + c * CH;
-        if (vb >= vend) return;
-        VC& ch = vt[vb / CH];
-        T fn[NF][CH], Mn[CH], Mo[CH];
-        maxw(ch, An, nl2, Un, Mn);
-        if constexpr (SHAK) maxw(ch, Ao, nlo2, Uo, Mo);
-        static_for<CH>([&](auto jc) {
-            constexpr int j = jc;
-            T MH = Mn[j];
-            T MB = MH * kbn;
-            if constexpr (SHAK) {
-                const T co = ch.u[j] - Uo, c2 = co * co;
-                const T t = pre * co * Mo[j];
-                if constexpr (NF == 1) {
-                    MH += t * (lo2 * c2 - T(5));
-                } else {
-                    MH += t * (lo2 * c2 + T(K) - T(5));
-                    MB += (t * kbo) * (lo2 * c2 + T(K) - T(3));
-                }
-            }
-            fn[0][j] = G[0][c * CH + j] * inv + MH * rTi;
-            if constexpr (NF == 2) fn[1][j] = G[1][c * CH + j] * inv + MB * rTi;
         });
-        if (upd) {
-            if (vb + CH <= vend) {
+
+        // ---- wavefront 0 adds the wavefronts' sums (fixed order) and runs the cells' step! chain;
+        // the others take its coefficients from LDS
+        T An, nl2, Un, inv, rTi, kbn, pre = 0, Ao = 0, nlo2 = 0, Uo = 0, lo2 = 0, kbo = 0;
+        T* coef = (T*)(lds + (size_t)NW * NR * kSweepLanes);  // [12][64]
+        if (NW > 1) {
+#pragma unroll
+            for (int m = 0; m < NR; m++) lds[(wave * NR + m) * kSweepLanes + lane] = acc[m];
+            __syncthreads();
+        }
+        if (wave == 0) {
+            double tot[NR];
+            if (NW > 1) {
+#pragma unroll
+                for (int m = 0; m < NR; m++) {
+                    double sm = 0.0;
+                    for (int w = 0; w < NW; w++) sm += lds[(w * NR + m) * kSweepLanes + lane];
+                    tot[m] = sm;
+                }
+            } else {
+#pragma unroll
+                for (int m = 0; m < NR; m++) tot[m] = acc[m];
+            }
+            double wo[3], po[3] = {1.0, 0.0, 1.0};
+            if constexpr (s == 0) {
+#pragma unroll
+                for (int m = 0; m < 3; m++) wo[m] = win[(size_t)kc * 3 + m];
+                if (SHAK) {
+#pragma unroll
+                    for (int m = 0; m < 3; m++) po[m] = S.prim[(size_t)kc * 3 + m];
+                }
+            } else {
+#pragma unroll
+                for (int m = 0; m < 3; m++) {
+                    wo[m] = wkeep[m];
+                    po[m] = pkeep[m];
+                }
+            }
+
+            // face moments of the lane's left face, and of its right face (the next lane's left)
+            double fwl[3];
+            if constexpr (FLUX == KFVS) {
+                fwl[0] = tot[0];
+                fwl[1] = tot[1];
+                fwl[2] = NF == 2 ? 0.5 * (tot[2] + tot[3]) : 0.5 * tot[2];
+            } else if constexpr (FLUX == KCU) {
+                fwl[0] = fw0[0] + m3 * tot[0];
+                fwl[1] = fw0[1] + m3 * tot[1];
+                fwl[2] = fw0[2] + m3 * 0.5 * (NF == 2 ? tot[2] + tot[3] : tot[2]);
+            } else {
+                fwl[0] = fw0[0] + (m3 * tot[0] - m4 * tot[1]);
+                fwl[1] = fw0[1] + (m3 * tot[2] - m4 * tot[3]);
+                fwl[2] = fw0[2] + (m3 * 0.5 * tot[4] - m4 * 0.5 * tot[5]);
+            }
+            if (bufL) {
+#pragma unroll
+                for (int m = 0; m < 3; m++) fwl[m] = S.bfw[bside * 3 + m];
+            }
+            double wn[3], pn[3];
+#pragma unroll
+            for (int m = 0; m < 3; m++) wn[m] = wo[m] + (fwl[m] - shl1(fwl[m])) * rdx64;
+
+            // the cell's step! coefficients (FP64 chain, lane-parallel)
+            conserve_prim<3>(wn, S.gam, pn);
+            const T rT = relax_ratio<T>(pn[0], pn[2], S.mu, S.om, dt);
+            inv = T(1) / (T(1) + rT);
+            rTi = rT * inv;
+            An = T(pn[0] * sqrt(pn[2] / kPi));
+            nl2 = T(-pn[2] * kLog2e);
+            Un = T(pn[1]);
+            kbn = T(K / (2.0 * pn[2]));
+            if (SHAK && shak) {
+                const double q = 0.5 * tot[NRAW];
+                pre = T(0.8 * (1.0 - S.Pr) * (po[2] * po[2]) / po[0] * q);
+                Ao = T(po[0] * sqrt(po[2] / kPi));
+                nlo2 = T(-po[2] * kLog2e);
+                Uo = T(po[1]);
+                lo2 = T(2.0 * po[2]);
+                kbo = T(K / (2.0 * po[2]));
+            }
+            if constexpr (!LAST) {
+                // the first of two steps: the cells it updates carry w, prim to the second
+#pragma unroll
+                for (int m = 0; m < 3; m++) {
+                    wkeep[m] = upd_mid ? wn[m] : wo[m];
+                    pkeep[m] = upd_mid ? pn[m] : po[m];
+                }
+            }
+            if (LAST && upd) {
+                const size_t r = (size_t)kc * 3;
+                const gptr<double> wout = to_global(S.w[in ^ 1]);
+                const gptr<double> gp = to_global(S.prim);
+#pragma unroll
+                for (int m = 0; m < 3; m++) {
+                    wout[r + m] = wn[m];
+                    gp[r + m] = pn[m];
+                }
+            }
+            if (LAST && rrow) {
+                // residual sums (solver_update.jl:86-88, update_boundary.jl:41-44): interior / boundary cells
+                double r[12];
+                const bool bnd = i == 1 || i == nx;
+#pragma unroll
+                for (int m = 0; m < 3; m++) {
+                    const double d2 = upd ? (wn[m] - wo[m]) * (wn[m] - wo[m]) : 0.0, a = upd ? fabs(wn[m]) : 0.0;
+                    r[m] = bnd ? 0.0 : d2;
+                    r[3 + m] = bnd ? 0.0 : a;
+                    r[6 + m] = bnd ? d2 : 0.0;
+                    r[9 + m] = bnd ? a : 0.0;
+                }
+                wave_allreduce<12>(r);
+                double y = 0.0;
+#pragma unroll
+                for (int m = 0; m < 12; m++)
+                    if (lane == m) y = r[m];
+                if (lane < 12) rrow[lane] = y;
+            }
+            if (NW > 1) {
+                const T cv[12] = {An, nl2, Un, inv, rTi, kbn, pre, Ao, nlo2, Uo, lo2, kbo};
+#pragma unroll
+                for (int m = 0; m < 12; m++) coef[m * kSweepLanes + lane] = cv[m];
+            }
+        }
+        if (NW > 1) {
+            __syncthreads();
+            if (wave != 0) {
+                An = coef[0 * kSweepLanes + lane];
+                nl2 = coef[1 * kSweepLanes + lane];
+                Un = coef[2 * kSweepLanes + lane];
+                inv = coef[3 * kSweepLanes + lane];
+                rTi = coef[4 * kSweepLanes + lane];
+                kbn = coef[5 * kSweepLanes + lane];
+                if (SHAK) {
+                    pre = coef[6 * kSweepLanes + lane];
+                    Ao = coef[7 * kSweepLanes + lane];
+                    nlo2 = coef[8 * kSweepLanes + lane];
+                    Uo = coef[9 * kSweepLanes + lane];
+                    lo2 = coef[10 * kSweepLanes + lane];
+                    kbo = coef[11 * kSweepLanes + lane];
+                }
+            }
+        }
+        if constexpr (!LAST) UoT = Un;  // the second step's Shakhov heat flux is about prim^{n+1}
+
+        // ---- pass 2: relaxation toward M (+ Shakhov) and the store of f^{n+1}
+        // M at the chunk's nodes: one exp per node, or -- FP32 on equally spaced nodes -- the product
+        // recurrence M(u + du) = M(u) R, R <- R Q with R = exp(-lam du (2 (u - U) + du)), Q = exp(-2 lam du^2),
+        // restarted every chunk of 8 nodes (at most ~30 roundings against the exp per node)
+        auto maxw = [&](VC& ch, T A, T nlam2, T U0, T (&M)[CH]) {
+            if (sizeof(T) == 4 && ch.du != T(0)) {
+                const T d = ch.du, c0 = ch.u[0] - U0;
+                T m = A * texp2(nlam2 * (c0 * c0));
+                T R = texp2(nlam2 * d * (T(2) * c0 + d));
+                const T Q = texp2(T(2) * nlam2 * (d * d));
                 static_for<CH>([&](auto jc) {
                     constexpr int j = jc;
-#pragma unroll
-                    for (int q = 0; q < NF; q++) bstore(fn[q][j], fout, voff, (q * nv + vb + j) * rowb);
+                    M[j] = m;
+                    if constexpr (j + 1 < CH) {
+                        m *= R;
+                        R *= Q;
+                    }
                 });
             } else {
                 static_for<CH>([&](auto jc) {
                     constexpr int j = jc;
-                    if (vb + j < vend) {
-#pragma unroll
-                        for (int q = 0; q < NF; q++) bstore(fn[q][j], fout, voff, (q * nv + vb + j) * rowb);
-                    }
+                    const T c = ch.u[j] - U0;
+                    M[j] = A * texp2(nlam2 * (c * c));
                 });
             }
-        }
+        };
+        static_for<VW / CH>([&](auto cc) {
+            constexpr int c = cc;
+            const int vb = vbeg + c * CH;
+            if (vb >= vend) return;
+            VC& ch = vt[vb / CH];
+            T fn[NF][CH], Mn[CH], Mo[CH];
+            maxw(ch, An, nl2, Un, Mn);
+            if constexpr (SHAK) maxw(ch, Ao, nlo2, Uo, Mo);
+            static_for<CH>([&](auto jc) {
+                constexpr int j = jc;
+                T MH = Mn[j];
+                T MB = MH * kbn;
+                if constexpr (SHAK) {
+                    const T co = ch.u[j] - Uo, c2 = co * co;
+                    const T t = pre * co * Mo[j];
+                    if constexpr (NF == 1) {
+                        MH += t * (lo2 * c2 - T(5));
+                    } else {
+                        MH += t * (lo2 * c2 + T(K) - T(5));
+                        MB += (t * kbo) * (lo2 * c2 + T(K) - T(3));
+                    }
+                }
+                fn[0][j] = G[0][c * CH + j] * inv + MH * rTi;
+                if constexpr (NF == 2) fn[1][j] = G[1][c * CH + j] * inv + MB * rTi;
+            });
+            if constexpr (LAST) {
+                if (upd) {
+                    if (vb + CH <= vend) {
+                        static_for<CH>([&](auto jc) {
+                            constexpr int j = jc;
+#pragma unroll
+                            for (int q = 0; q < NF; q++) bstore<KBG_SWEEP_STORE_AUX>(fn[q][j], fout, voff, (q * nv + vb + j) * rowb);
+                        });
+                    } else {
+                        static_for<CH>([&](auto jc) {
+                            constexpr int j = jc;
+                            if (vb + j < vend) {
+#pragma unroll
+                                for (int q = 0; q < NF; q++) bstore<KBG_SWEEP_STORE_AUX>(fn[q][j], fout, voff, (q * nv + vb + j) * rowb);
+                            }
+                        });
+                    }
+                }
+            } else {
+                // f^{n+1} becomes the second step's input; cells the first step left alone keep f^n
+                static_for<CH>([&](auto jc) {
+                    constexpr int j = jc;
+#pragma unroll
+                    for (int q = 0; q < NF; q++) G[q][c * CH + j] = fn[q][j];
+                });
+                if (keep_any) {
+                    static_for<CH>([&](auto jc) {
+                        constexpr int j = jc;
+                        if (vb + j < vend) {
+#pragma unroll
+                            for (int q = 0; q < NF; q++) {
+                                const T f0 = bload((T*)nullptr, fin, voff, (q * nv + vb + j) * rowb);
+                                G[q][c * CH + j] = upd_mid ? G[q][c * CH + j] : f0;
+                            }
+                        }
+                    });
+                }
+            }
+        });
+
     });
 }
 
-template <class T, int NF, int FLUX, int LIMK>
+template <class T, int NF, int FLUX, int LIMK, int NSTEP>
 KBG_D void sweep_coll(const Solve1D& S, const WorkItem& wi, int par, int lane, int wave, int NW, double* lds, double* rrow) {
-    if (S.coll == SHAKHOV) sweep1d<T, NF, FLUX, LIMK, true>(S, wi, par, lane, wave, NW, lds, rrow);
-    else sweep1d<T, NF, FLUX, LIMK, false>(S, wi, par, lane, wave, NW, lds, rrow);
+    if (S.coll == SHAKHOV) sweep1d<T, NF, FLUX, LIMK, true, NSTEP>(S, wi, par, lane, wave, NW, lds, rrow);
+    else sweep1d<T, NF, FLUX, LIMK, false, NSTEP>(S, wi, par, lane, wave, NW, lds, rrow);
+}
+template <class T, int NF, int FLUX, int NSTEP>
+KBG_D void sweep_lim(const Solve1D& S, const WorkItem& wi, int par, int lane, int wave, int NW, double* lds, double* rrow) {
+    if (!S.interp2) sweep_coll<T, NF, FLUX, LIM_NONE, NSTEP>(S, wi, par, lane, wave, NW, lds, rrow);
+    else if (S.limiter == VANLEER) sweep_coll<T, NF, FLUX, LIM_VANLEER, NSTEP>(S, wi, par, lane, wave, NW, lds, rrow);
+    else sweep_coll<T, NF, FLUX, LIM_GENERIC, NSTEP>(S, wi, par, lane, wave, NW, lds, rrow);
 }
 
 // LDS of one item (doubles): the wavefronts' moment sums and wavefront 0's cell coefficients
@@ -588,9 +670,6 @@ template <class T, int NF, int FLUX> constexpr size_t sweep_lds_item(int nw) {
 
 // occupancy target: 4 wavefronts per SIMD (<= 128 VGPRs) for the FP32 one-distribution sweeps,
 // whatever the register file allows for the others
-#ifndef KBG_SWEEP_WPE_F32_1F
-#define KBG_SWEEP_WPE_F32_1F 4
-#endif
 template <class T, int NF, int FLUX> constexpr int sweep_waves_per_eu() { return sizeof(T) == 4 && NF == 1 ? KBG_SWEEP_WPE_F32_1F : 1; }
 
 // One workgroup per pack; packs of one XCD (workgroup ids congruent mod 8) are a contiguous run,
@@ -599,11 +678,14 @@ template <class T, int NF, int FLUX> constexpr int sweep_waves_per_eu() { return
 template <class T, int NF, int FLUX>
 __global__ __launch_bounds__(64 * kSweepMaxWaves) __attribute__((amdgpu_waves_per_eu(sweep_waves_per_eu<T, NF, FLUX>()))) void k_sweep1d(
     const Solve1D* __restrict__ solves, const WorkItem* __restrict__ items, const Pack* __restrict__ packs, int npacks, int par,
-    double* __restrict__ rpart) {
+    double* __restrict__ rpart, int nstep) {
     extern __shared__ double kbg_sweep_lds[];
     const int G = gridDim.x, b = blockIdx.x;
     const int x = b & 7, q8 = G >> 3, r8 = G & 7;
-    const int pk = x * q8 + (x < r8 ? x : r8) + (b >> 3);
+    // each XCD walks its run of packs forwards on even steps and backwards on odd ones, so a step
+    // first reads the rows the previous step wrote last (still in the memory-side cache)
+    const int nx8 = q8 + (x < r8 ? 1 : 0);
+    const int pk = x * q8 + (x < r8 ? x : r8) + ((KBG_SWEEP_ZIGZAG && (par & 1)) ? nx8 - 1 - (b >> 3) : (b >> 3));
     if (pk >= npacks) return;
     const Pack P = packs[pk];
     const int lane = threadIdx.x & 63;
@@ -611,10 +693,10 @@ __global__ __launch_bounds__(64 * kSweepMaxWaves) __attribute__((amdgpu_waves_pe
     const int NW = P.nw;
     const int sub = wv / NW, wave = wv - sub * NW;
     if (sub >= P.count) {
-        // an unused slot still takes part in the item barriers (two when nw > 1)
+        // an unused slot still takes part in the item barriers (two per time step when nw > 1)
         if (NW > 1) {
-            __syncthreads();
-            __syncthreads();
+            const int nb = (FLUX == KFVS && nstep == 2) ? 4 : 2;
+            for (int b2 = 0; b2 < nb; b2++) __syncthreads();
         }
         return;
     }
@@ -623,19 +705,23 @@ __global__ __launch_bounds__(64 * kSweepMaxWaves) __attribute__((amdgpu_waves_pe
     const Solve1D& S = solves[wi.solve];
     double* lds = kbg_sweep_lds + (size_t)sub * sweep_lds_item<T, NF, FLUX>(NW);
     double* rrow = rpart ? rpart + (size_t)it * 12 : nullptr;
-    if (!S.interp2) sweep_coll<T, NF, FLUX, LIM_NONE>(S, wi, par, lane, wave, NW, lds, rrow);
-    else if (S.limiter == VANLEER) sweep_coll<T, NF, FLUX, LIM_VANLEER>(S, wi, par, lane, wave, NW, lds, rrow);
-    else sweep_coll<T, NF, FLUX, LIM_GENERIC>(S, wi, par, lane, wave, NW, lds, rrow);
+    if constexpr (FLUX == KFVS) {
+        if (nstep == 2) {
+            sweep_lim<T, NF, FLUX, 2>(S, wi, par, lane, wave, NW, lds, rrow);
+            return;
+        }
+    }
+    sweep_lim<T, NF, FLUX, 1>(S, wi, par, lane, wave, NW, lds, rrow);
 }
 
 template <class T, int NF, int FLUX>
 hipError_t launch_sweep_t(int wg_waves, const Solve1D* d_solves, const WorkItem* d_items, const Pack* d_packs, int npacks,
-                          int par, double* d_rpart, hipStream_t st) {
+                          int par, double* d_rpart, int nstep, hipStream_t st) {
     // the largest LDS a pack of this launch can need: wg_waves / nw items of nw wavefronts each
     size_t lds = 0;
     for (int nw = 2; nw <= wg_waves; nw *= 2) lds = std::max(lds, (size_t)(wg_waves / nw) * sweep_lds_item<T, NF, FLUX>(nw));
     hipLaunchKernelGGL((k_sweep1d<T, NF, FLUX>), dim3(npacks), dim3(64 * wg_waves), lds * sizeof(double), st, d_solves, d_items,
-                       d_packs, npacks, par, d_rpart);
+                       d_packs, npacks, par, d_rpart, nstep);
     return hipGetLastError();
 }
 

@@ -4,18 +4,18 @@
 namespace kbg {
 
 hipError_t launch_sweep1d_f64(int nf, int flux, int wg_waves, const Solve1D* d_solves, const WorkItem* d_items,
-                              const Pack* d_packs, int npacks, int par, double* d_rpart, hipStream_t st);
+                              const Pack* d_packs, int npacks, int par, double* d_rpart, int nstep, hipStream_t st);
 
 hipError_t launch_sweep1d(bool fp64, int nf, int flux, int wg_waves, const Solve1D* d_solves, const WorkItem* d_items,
-                          const Pack* d_packs, int npacks, int par, double* d_rpart, hipStream_t st) {
+                          const Pack* d_packs, int npacks, int par, double* d_rpart, int nstep, hipStream_t st) {
     if (npacks <= 0) return hipSuccess;
     if (wg_waves < 1 || wg_waves > kSweepMaxWaves) return hipErrorInvalidValue;
-    if (fp64) return launch_sweep1d_f64(nf, flux, wg_waves, d_solves, d_items, d_packs, npacks, par, d_rpart, st);
-    if (nf == 1 && flux == KFVS) return launch_sweep_t<float, 1, KFVS>(wg_waves, d_solves, d_items, d_packs, npacks, par, d_rpart, st);
-    if (nf == 1 && flux == KCU) return launch_sweep_t<float, 1, KCU>(wg_waves, d_solves, d_items, d_packs, npacks, par, d_rpart, st);
-    if (nf == 2 && flux == KFVS) return launch_sweep_t<float, 2, KFVS>(wg_waves, d_solves, d_items, d_packs, npacks, par, d_rpart, st);
-    if (nf == 2 && flux == KCU) return launch_sweep_t<float, 2, KCU>(wg_waves, d_solves, d_items, d_packs, npacks, par, d_rpart, st);
-    if (nf == 2 && flux == UGKS) return launch_sweep_t<float, 2, UGKS>(wg_waves, d_solves, d_items, d_packs, npacks, par, d_rpart, st);
+    if (fp64) return launch_sweep1d_f64(nf, flux, wg_waves, d_solves, d_items, d_packs, npacks, par, d_rpart, nstep, st);
+    if (nf == 1 && flux == KFVS) return launch_sweep_t<float, 1, KFVS>(wg_waves, d_solves, d_items, d_packs, npacks, par, d_rpart, nstep, st);
+    if (nf == 1 && flux == KCU) return launch_sweep_t<float, 1, KCU>(wg_waves, d_solves, d_items, d_packs, npacks, par, d_rpart, nstep, st);
+    if (nf == 2 && flux == KFVS) return launch_sweep_t<float, 2, KFVS>(wg_waves, d_solves, d_items, d_packs, npacks, par, d_rpart, nstep, st);
+    if (nf == 2 && flux == KCU) return launch_sweep_t<float, 2, KCU>(wg_waves, d_solves, d_items, d_packs, npacks, par, d_rpart, nstep, st);
+    if (nf == 2 && flux == UGKS) return launch_sweep_t<float, 2, UGKS>(wg_waves, d_solves, d_items, d_packs, npacks, par, d_rpart, nstep, st);
     return hipErrorInvalidValue;
 }
 

@@ -4,12 +4,12 @@
 namespace kbg {
 
 hipError_t launch_sweep1d_f64(int nf, int flux, int wg_waves, const Solve1D* d_solves, const WorkItem* d_items,
-                              const Pack* d_packs, int npacks, int par, double* d_rpart, hipStream_t st) {
-    if (nf == 1 && flux == KFVS) return launch_sweep_t<double, 1, KFVS>(wg_waves, d_solves, d_items, d_packs, npacks, par, d_rpart, st);
-    if (nf == 1 && flux == KCU) return launch_sweep_t<double, 1, KCU>(wg_waves, d_solves, d_items, d_packs, npacks, par, d_rpart, st);
-    if (nf == 2 && flux == KFVS) return launch_sweep_t<double, 2, KFVS>(wg_waves, d_solves, d_items, d_packs, npacks, par, d_rpart, st);
-    if (nf == 2 && flux == KCU) return launch_sweep_t<double, 2, KCU>(wg_waves, d_solves, d_items, d_packs, npacks, par, d_rpart, st);
-    if (nf == 2 && flux == UGKS) return launch_sweep_t<double, 2, UGKS>(wg_waves, d_solves, d_items, d_packs, npacks, par, d_rpart, st);
+                              const Pack* d_packs, int npacks, int par, double* d_rpart, int nstep, hipStream_t st) {
+    if (nf == 1 && flux == KFVS) return launch_sweep_t<double, 1, KFVS>(wg_waves, d_solves, d_items, d_packs, npacks, par, d_rpart, nstep, st);
+    if (nf == 1 && flux == KCU) return launch_sweep_t<double, 1, KCU>(wg_waves, d_solves, d_items, d_packs, npacks, par, d_rpart, nstep, st);
+    if (nf == 2 && flux == KFVS) return launch_sweep_t<double, 2, KFVS>(wg_waves, d_solves, d_items, d_packs, npacks, par, d_rpart, nstep, st);
+    if (nf == 2 && flux == KCU) return launch_sweep_t<double, 2, KCU>(wg_waves, d_solves, d_items, d_packs, npacks, par, d_rpart, nstep, st);
+    if (nf == 2 && flux == UGKS) return launch_sweep_t<double, 2, UGKS>(wg_waves, d_solves, d_items, d_packs, npacks, par, d_rpart, nstep, st);
     return hipErrorInvalidValue;
 }
 

@@ -100,8 +100,10 @@ struct Pack {
 
 // ---- launchers (kbg_sweep.h, kbg_1d.hip, kbg_2d.hip) --------------------------------------------------------
 // All return hipSuccess or the launch error; `fp64` selects the storage type.
+// nstep: time steps per launch, 1, or 2 for KFVS solves without boundary-face buffers (items of at most
+// kSweepOut2 = 56 cells); par counts launches, not steps
 hipError_t launch_sweep1d(bool fp64, int nf, int flux, int wg_waves, const Solve1D* d_solves, const WorkItem* d_items,
-                          const Pack* d_packs, int npacks, int par, double* d_rpart, hipStream_t st);
+                          const Pack* d_packs, int npacks, int par, double* d_rpart, int nstep, hipStream_t st);
 // velocities per wavefront of the sweep for a storage type / distribution count (a multiple of 8),
 // and the largest velocity grid it takes (8 wavefronts)
 #ifndef KBG_SWEEP_VW32

@@ -426,6 +426,10 @@ class Context:
         """Replay batch steps through HIP graphs (kbg_ctx_set_graphs; default on)."""
         _lib.check(self.lib.kbg_ctx_set_graphs(self.handle, 1 if enable else 0))
 
+    def set_sweep_steps(self, steps: int):
+        """Time steps per fused 1D sweep launch, 1 or 2 (kbg_ctx_set_sweep_steps; default 2)."""
+        _lib.check(self.lib.kbg_ctx_set_sweep_steps(self.handle, int(steps)))
+
     def schedule(self) -> tuple[int, int]:
         a, b = C.c_int(), C.c_int()
         _lib.check(self.lib.kbg_ctx_get_schedule(self.handle, C.byref(a), C.byref(b)))

@@ -496,3 +496,51 @@ def test_graph_replay_matches_eager_launches(gpu_ctx):
     gpu_ctx.set_graphs(False)
     for _, _, d, _ in A + B:
         d.close()
+
+
+# ---------------------------------------------------------------- two time steps per sweep launch
+
+_TWO_CASES = [
+    # (config overrides, precision)
+    (dict(nx=70, collision="shakhov", knudsen=1e-2), "fp64"),
+    (dict(nx=90, nxg=2, collision="bgk", knudsen=1e-3), "fp64"),          # ghost cells: kept f^n
+    (dict(nx=64, nu=200, vMeshType="rectangle", collision="shakhov"), "fp32"),  # 4 wavefronts
+    (dict(nx=120, nu=512, vMeshType="rectangle", collision="bgk"), "fp32"),    # 8 wavefronts
+    (dict(nx=50, nu=24, vMeshType="rectangle"), "fp32"),                   # 1 wavefront
+    (dict(nx=60, limiter="minmod", collision="shakhov"), "fp64"),
+    (dict(nx=60, interpOrder=1), "fp32"),
+]
+
+
+@pytest.mark.parametrize("case", range(len(_TWO_CASES)))
+@pytest.mark.parametrize("nsteps", [2, 7])
+def test_two_steps_per_sweep_launch_bitwise(gpu_ctx, case, nsteps):
+    """The fused sweep advancing two steps per pass (4-cell halo, f^{n+1} kept in registers) gives the
+    same bits as one step per launch, for odd and even step counts and several segment lengths."""
+    over, prec = _TWO_CASES[case]
+    ks = kb.SolverSet(kb.configs.sod_1d1f1v(**over))
+    ctr, face = kb.init_fvm(ks)
+    dt = kb.timestep(ks, ctr)
+    one_ctx = kb.Context(0)
+    one_ctx.set_sweep_steps(1)
+    old = gpu_ctx.schedule()
+    try:
+        for seg in (56, 13):
+            gpu_ctx.set_schedule(seg, 0)
+            one_ctx.set_schedule(seg, 0)
+            a = kb.DeviceSolve(ks, ctr, face, prec, gpu_ctx)
+            b = kb.DeviceSolve(ks, ctr, face, prec, one_ctx)
+            ra, rb = np.zeros(3), np.zeros(3)
+            a.step(dt, nsteps, ra)
+            b.step(dt, nsteps, rb)
+            a.step(dt, nsteps)
+            b.step(dt, nsteps)
+            ca, cb = copy.deepcopy(ctr), copy.deepcopy(ctr)
+            a.download(ca)
+            b.download(cb)
+            assert np.array_equal(ca.f, cb.f) and np.array_equal(ca.w, cb.w) and np.array_equal(ca.prim, cb.prim)
+            assert np.array_equal(ra, rb)
+            a.close()
+            b.close()
+    finally:
+        gpu_ctx.set_schedule(*old)
