@@ -13,7 +13,8 @@ Printed (one JSON line on stdout):
                  against the FP64 single-solve results after the same steps
   roofline       HBM roofline of the sweep kernels: algorithmic bytes per step (8 B per
                  cell x velocity + 48 B per cell) / device time per step (HIP events on the
-                 library's stream around the step loop)
+                 library's stream around the step loop); the sweep advances two steps per pass
+                 over f, so its measured HBM traffic ("traffic", per step) is below that figure
   cpu_baseline   the FP64 CPU oracle (OpenMP) on a bounded sample of the same instances
 
     python bench.py [--steps K] [--warmup W] [--instances N] [--no-cpu] [--no-fp64]
@@ -326,7 +327,9 @@ def main():
         "serving": serve,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_sweep1d<float,1F,KFVS>: one launch per wavefront count (1..8) per step",
+                     "kernel": "k_sweep1d<float,1F,KFVS>: two time steps per launch, one launch per stream "
+                               "(4 concurrent); achieved = per-update algorithmic bytes x updates / device time",
+                     "steps_per_launch": a.sweep_steps,
                      "algorithmic_bytes_per_step": alg_bytes_step, "device_ms_per_step": ms_step},
         "cpu_baseline": cpu,
     }
