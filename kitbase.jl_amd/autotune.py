@@ -26,7 +26,7 @@ def signature(devs) -> str:
     return hashlib.sha1(repr(parts).encode()).hexdigest()[:16]
 
 
-def tune(ctx, devs, dts, candidates=None, steps: int = 6, repeat: int = 2, cache: str | Path | None = None):
+def tune(ctx, devs, dts, candidates=None, steps: int = 20, repeat: int = 2, cache: str | Path | None = None):
     """Returns (best (seg_cells, streams), {candidate: ms per step}, steps taken)."""
     from .solver import batch_step
 
