@@ -177,6 +177,21 @@ hipError_t launch_face2d(bool inline_sw, int flux, const Solve2D* d_solves, cons
 // ---- the fused step: face fluxes with inline slopes, then the cell update ------------------------
 
 constexpr int kMaxWaves2 = 16;  // wavefronts per workgroup at most (1024 threads)
+// velocities per thread of the face-flux kernel (a multiple of 4: its per-face set-up is shared by
+// more velocities than the cell kernel's 4)
+#ifndef KBG_FLUX2D_VPT32
+#define KBG_FLUX2D_VPT32 8
+#endif
+#ifndef KBG_FLUX2D_VPT64
+#define KBG_FLUX2D_VPT64 8
+#endif
+template <class T> constexpr int flux2d_vpt() { return sizeof(T) == 4 ? KBG_FLUX2D_VPT32 : KBG_FLUX2D_VPT64; }
+template <class T> constexpr int flux2d_max_threads() { return step2d_max_threads<T>() * step2d_vpt<T>() / flux2d_vpt<T>(); }
+// threads of the face-flux workgroup for a group whose cell workgroups have `threads` (4 velocities
+// per thread, 64 ceil(nv / 256)): 64 ceil(threads / (16 VPT)), which covers the same velocities
+constexpr int flux2d_threads(bool fp64, int threads) {
+    return (threads + 16 * (fp64 ? KBG_FLUX2D_VPT64 : KBG_FLUX2D_VPT32) - 1) / (16 * (fp64 ? KBG_FLUX2D_VPT64 : KBG_FLUX2D_VPT32)) * 64;
+}
 
 // XCD-aware order: the workgroups of one XCD (ids congruent mod 8) take a contiguous run of items,
 // so neighbouring faces and cells are read through the same L2
@@ -195,9 +210,9 @@ KBG_D int xcd_order() {
 // from zero mass flux in a first pass and writes the wall flux in a second.  A :fix boundary face
 // keeps its stored flux.
 template <class T, int FLUX>
-__global__ __launch_bounds__(step2d_max_threads<T>()) void k_flux2d(const Solve2D* __restrict__ solves, const int* __restrict__ fstart,
+__global__ __launch_bounds__(flux2d_max_threads<T>()) void k_flux2d(const Solve2D* __restrict__ solves, const int* __restrict__ fstart,
                                                              int nsolves, int nfaces1) {
-    constexpr int VPT = step2d_vpt<T>();
+    constexpr int VPT = flux2d_vpt<T>();
     constexpr int E = sizeof(T);
     __shared__ double red[kMaxWaves2 * 16];
     const int g = xcd_order();
@@ -660,7 +675,7 @@ hipError_t launch_flux2d(bool fp64, int flux, int threads, const Solve2D* d_solv
     if (nfaces <= 0) return hipSuccess;
     if (threads < 64 || threads > (fp64 ? step2d_max_threads<double>() : step2d_max_threads<float>()) || threads % 64)
         return hipErrorInvalidValue;
-    const dim3 g(nfaces), b(threads);
+    const dim3 g(nfaces), b(flux2d_threads(fp64, threads));
     if (fp64) {
         if (flux == KFVS) hipLaunchKernelGGL((k_flux2d<double, KFVS>), g, b, 0, st, d_solves, d_fstart, nsolves, nfaces);
         else if (flux == KCU) hipLaunchKernelGGL((k_flux2d<double, KCU>), g, b, 0, st, d_solves, d_fstart, nsolves, nfaces);

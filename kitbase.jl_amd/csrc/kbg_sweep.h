@@ -467,6 +467,10 @@ KBG_D void sweep1d(const Solve1D& S, const WorkItem& wi, int par, int lane, int 
                 fwl[1] = fw0[1] + (m3 * tot[2] - m4 * tot[3]);
                 fwl[2] = fw0[2] + (m3 * 0.5 * tot[4] - m4 * 0.5 * tot[5]);
             }
+            // pinned: otherwise the override below becomes one load through a pointer that selects
+            // between bfw and the moment sums, which sends the sums to scratch memory
+#pragma unroll
+            for (int m = 0; m < 3; m++) asm volatile("" : "+v"(fwl[m]));
             if (bufL) {
 #pragma unroll
                 for (int m = 0; m < 3; m++) fwl[m] = S.bfw[bside * 3 + m];
