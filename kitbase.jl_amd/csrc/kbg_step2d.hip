@@ -17,8 +17,9 @@
 //
 // Each face flux is evaluated once (the flux is the VALU-heavy part of the step: about 100 FP32
 // operations per face and velocity for UGKS); f is read by the faces through L2 and written once by
-// the cells.  Workgroups are 64 ceil(nv / 256) threads, 4 velocities per thread, so grids from 16 to
-// 4096 (FP32) or 2048 (FP64) velocities keep every lane busy.
+// the cells.  Cell workgroups are 64 ceil(nv / 256) threads, 4 velocities per thread; face
+// workgroups 64 ceil(nv / 512) threads, 8 velocities per thread (more velocities share the per-face
+// set-up); grids of up to 4096 (FP32) or 2048 (FP64) velocities take this step.
 //
 // Reference: evolve! solver_evolution.jl:608-679 with flux_kfvs.jl:98-135 / :654-697,
 // flux_kcu.jl:149-191 / :898-967, flux_ugks.jl:121-253; reconstruct! solver_reconstruction.jl:475-620
