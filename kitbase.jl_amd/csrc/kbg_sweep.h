@@ -291,7 +291,9 @@ KBG_D void sweep1d(const Solve1D& S, const WorkItem& wi, int par, int lane, int 
                 if constexpr (SS == 0) f = fr[q][j];
                 else f = G[q][c * CH + j];  // the first step's f^{n+1}, still in registers
                 fq[q] = f;
-                sq[q] = sweep_limit<T, LIMK, HALF>(lk, (f - shr1(f)) * rL, (shl1(f) - f) * rR);
+                // the right difference f[L+1] - f[L] is the next lane's left one, bit for bit
+                const T d = f - shr1(f);
+                sq[q] = sweep_limit<T, LIMK, HALF>(lk, d * rL, shl1(d) * rR);
             }
             // upwind state and slope of the left face (flux_kfvs.jl:238-264, :303-340)
 #pragma unroll
@@ -484,13 +486,15 @@ KBG_D void sweep1d(const Solve1D& S, const WorkItem& wi, int par, int lane, int 
             const T rT = relax_ratio<T>(pn[0], pn[2], S.mu, S.om, dt);
             inv = T(1) / (T(1) + rT);
             rTi = rT * inv;
-            An = T(pn[0] * sqrt(pn[2] / kPi));
+            // the relaxation weight rT / (1 + rT) is folded into the equilibrium amplitude (and the
+            // Shakhov prefactor below), one multiply per velocity less in pass 2
+            An = T(pn[0] * sqrt(pn[2] / kPi)) * rTi;
             nl2 = T(-pn[2] * kLog2e);
             Un = T(pn[1]);
             kbn = T(K / (2.0 * pn[2]));
             if (SHAK && shak) {
                 const double q = 0.5 * tot[NRAW];
-                pre = T(0.8 * (1.0 - S.Pr) * (po[2] * po[2]) / po[0] * q);
+                pre = T(0.8 * (1.0 - S.Pr) * (po[2] * po[2]) / po[0] * q) * rTi;
                 Ao = T(po[0] * sqrt(po[2] / kPi));
                 nlo2 = T(-po[2] * kLog2e);
                 Uo = T(po[1]);
@@ -609,8 +613,8 @@ KBG_D void sweep1d(const Solve1D& S, const WorkItem& wi, int par, int lane, int 
                         MB += (t * kbo) * (lo2 * c2 + T(K) - T(3));
                     }
                 }
-                fn[0][j] = G[0][c * CH + j] * inv + MH * rTi;
-                if constexpr (NF == 2) fn[1][j] = G[1][c * CH + j] * inv + MB * rTi;
+                fn[0][j] = G[0][c * CH + j] * inv + MH;  // MH, MB carry the weight rT / (1 + rT)
+                if constexpr (NF == 2) fn[1][j] = G[1][c * CH + j] * inv + MB;
             });
             if constexpr (LAST) {
                 if (upd) {
