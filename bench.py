@@ -327,8 +327,9 @@ def main():
         "serving": serve,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_sweep1d<float,1F,KFVS>: two time steps per launch, one launch per stream "
-                               "(4 concurrent); achieved = per-update algorithmic bytes x updates / device time",
+                     "kernel": f"k_sweep1d<float,1F,KFVS>: {a.sweep_steps} time step(s) per launch, one launch "
+                               f"per stream ({ctx.schedule()[1] or 4} concurrent streams, {ctx.schedule()[0]} cells "
+                               "per segment); achieved = per-update algorithmic bytes x updates / device time",
                      "steps_per_launch": a.sweep_steps,
                      "algorithmic_bytes_per_step": alg_bytes_step, "device_ms_per_step": ms_step},
         "cpu_baseline": cpu,
