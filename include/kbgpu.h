@@ -99,6 +99,12 @@ int kbg_ctx_set_graphs(kbg_ctx *ctx, int enable);
  * one write of f per two steps, bit-identical results; 1 always launches one step at a time.
  * No counterpart in KitBase. */
 int kbg_ctx_set_sweep_steps(kbg_ctx *ctx, int steps);
+/* Most velocities per wavefront of the fused 1D sweep: 0 (default) takes the kernel's full width
+ * (64 FP32 velocities of one distribution); 8, 16 or 32 spread each segment's velocity grid over
+ * more wavefronts (at most 8), which shortens the pass of small, latency-bound batches (serving)
+ * at some cost in throughput on large ones.  A solve's wavefront count depends only on its velocity
+ * count and this setting, so its results are the same in any batch.  No counterpart in KitBase. */
+int kbg_ctx_set_sweep_vw(kbg_ctx *ctx, int vw);
 /* HIP events on the context stream, for timing: record slot (0..15), elapsed ms between slots.
  * kbg_step / kbg_batch_step record slots KBG_EV_STEPS_BEGIN and KBG_EV_STEPS_END around their
  * time-step loop (after the descriptor upload, before the residual download), so the elapsed

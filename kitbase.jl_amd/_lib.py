@@ -57,6 +57,7 @@ _PROTOS = {
     "kbg_ctx_get_schedule": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "kbg_ctx_set_graphs": (C.c_int, [C.c_void_p, C.c_int]),
     "kbg_ctx_set_sweep_steps": (C.c_int, [C.c_void_p, C.c_int]),
+    "kbg_ctx_set_sweep_vw": (C.c_int, [C.c_void_p, C.c_int]),
     "kbg_ctx_event_record": (C.c_int, [C.c_void_p, C.c_int]),
     "kbg_ctx_event_elapsed": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_float)]),
     "kbg_solve_create": (C.c_int, [C.c_void_p, C.POINTER(KbgDesc), C.POINTER(C.c_void_p)]),

@@ -66,6 +66,7 @@ struct kbg_ctx {
     int* d_pref2 = nullptr;
     size_t pref2_cap = 0;
     int sweep_steps = 2;  // time steps per fused 1D sweep launch where the solves allow two
+    int sweep_vw = 0;     // most velocities per sweep wavefront (0: the kernel's width)
     // HIP graphs of a batch's steps: [0] a step pair (parities 0, 1), [1] one step (parity 0),
     // captured for the launch plan whose signature is gsig
     bool graphs = false;
@@ -451,14 +452,14 @@ int sync_boundary_faces(kbg_solve* s) {
 }
 
 // segments of the stepped cells, in order along x (neighbouring segments share halo rows)
-void build_items(kbg_solve* s, int seg, int solve_index, std::vector<WorkItem>& out) {
+void build_items(kbg_solve* s, int seg, int solve_index, int nw, std::vector<WorkItem>& out) {
     const Solve1D& S = s->h1;
     for (int c0 = S.upd_lo; c0 <= S.upd_hi; c0 += seg) {
         WorkItem wi;
         wi.solve = solve_index;
         wi.c0 = c0;
         wi.c1 = std::min(c0 + seg - 1, S.upd_hi);
-        wi.nw = s->sweep_nw;
+        wi.nw = nw;
         out.push_back(wi);
     }
 }
@@ -564,6 +565,13 @@ int kbg_ctx_set_sweep_steps(kbg_ctx* ctx, int steps) {
     if (!ctx) return fail(KBG_ERR_ARG, "ctx is NULL");
     if (steps != 1 && steps != 2) return fail(KBG_ERR_ARG, "sweep steps must be 1 or 2");
     ctx->sweep_steps = steps;
+    return KBG_OK;
+}
+
+int kbg_ctx_set_sweep_vw(kbg_ctx* ctx, int vw) {
+    if (!ctx) return fail(KBG_ERR_ARG, "ctx is NULL");
+    if (vw != 0 && vw != 8 && vw != 16 && vw != 32 && vw != 64) return fail(KBG_ERR_ARG, "sweep_vw must be 0, 8, 16, 32 or 64");
+    ctx->sweep_vw = vw;
     return KBG_OK;
 }
 
@@ -1105,9 +1113,15 @@ int kbg_batch_step(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double* 
             desc_of[i] = (int)descs.size();
             descs.push_back(s->h1);
             const size_t i0 = items.size();
-            build_items(s, seg, desc_of[i], items);
-            for (size_t it = i0; it < items.size(); it++) by_nw[s->sweep_nw].push_back((int)it);
-            L.wg_waves = std::max(L.wg_waves, s->sweep_nw);
+            // wavefronts per item: the kernel's width, or more when the context caps the velocities
+            // per wavefront (a function of the solve and the context only, so a solve's sums keep
+            // their order in any batch)
+            int nw = s->sweep_nw;
+            if (ctx->sweep_vw > 0)
+                while (nw < kbg::kSweepMaxWaves && (s->d.nv + nw - 1) / nw > ctx->sweep_vw) nw *= 2;
+            build_items(s, seg, desc_of[i], nw, items);
+            for (size_t it = i0; it < items.size(); it++) by_nw[nw].push_back((int)it);
+            L.wg_waves = std::max(L.wg_waves, nw);
         }
         // workgroups of at least 4 wavefronts: side-by-side items of the same nw fill them
         L.wg_waves = std::max(L.wg_waves, std::min(4, kbg::kSweepMaxWaves));

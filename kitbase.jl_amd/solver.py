@@ -430,6 +430,11 @@ class Context:
         """Time steps per fused 1D sweep launch, 1 or 2 (kbg_ctx_set_sweep_steps; default 2)."""
         _lib.check(self.lib.kbg_ctx_set_sweep_steps(self.handle, int(steps)))
 
+    def set_sweep_vw(self, vw: int):
+        """Most velocities per fused-sweep wavefront, 0 (full width), 8, 16 or 32 (kbg_ctx_set_sweep_vw):
+        smaller values shorten the passes of small, latency-bound batches."""
+        _lib.check(self.lib.kbg_ctx_set_sweep_vw(self.handle, int(vw)))
+
     def schedule(self) -> tuple[int, int]:
         a, b = C.c_int(), C.c_int()
         _lib.check(self.lib.kbg_ctx_get_schedule(self.handle, C.byref(a), C.byref(b)))

@@ -374,6 +374,46 @@ def test_batch_equals_individual_and_oracle(gpu_ctx):
         check_macro(c2.prim, co.prim, 1e-4)
 
 
+@pytest.mark.parametrize("vw", [8, 16, 32])
+def test_sweep_velocity_cap(gpu_ctx, vw):
+    """kbg_ctx_set_sweep_vw: the segment's velocity grid spread over more wavefronts (up to 8) gives
+    the oracle's moments, agrees with the full-width kernel to FP32 rounding of the FP64 sums, and a
+    solve's result is the same alone and in a batch bit for bit (its wavefront count does not depend
+    on the batch)."""
+    insts = kb.configs.batch_instances(6, seed=11)
+    for i, nv in zip(insts, (33, 64, 100, 129, 200, 256)):
+        i["nx"] = min(i["nx"], 120)
+        i["nu"] = nv
+    steps = 7
+    made = []
+    for inst in insts:
+        ks, ctr, face, dt = _instance(inst, "fp32", gpu_ctx)
+        co, fo = copy.deepcopy(ctr), copy.deepcopy(face)
+        oracle.OracleSolve(ks, co, fo).run(dt, steps)
+        made.append((ks, ctr, face, dt, co))
+    try:
+        gpu_ctx.set_sweep_vw(0)
+        full = [kb.DeviceSolve(ks, ctr, face, "fp32", gpu_ctx) for ks, ctr, face, dt, co in made]
+        kb.batch_step(full, [m[3] for m in made], steps, ctx=gpu_ctx)
+        gpu_ctx.set_sweep_vw(vw)
+        batch = [kb.DeviceSolve(ks, ctr, face, "fp32", gpu_ctx) for ks, ctr, face, dt, co in made]
+        kb.batch_step(batch, [m[3] for m in made], steps, ctx=gpu_ctx)
+        for (ks, ctr, face, dt, co), f, b in zip(made, full, batch):
+            alone = kb.DeviceSolve(ks, ctr, face, "fp32", gpu_ctx)
+            alone.step(dt, steps)
+            ca, cb, cf = copy.deepcopy(ctr), copy.deepcopy(ctr), copy.deepcopy(ctr)
+            alone.download(ca)
+            b.download(cb)
+            f.download(cf)
+            assert np.array_equal(ca.f, cb.f) and np.array_equal(ca.w, cb.w), (vw, ks.vs.nv)
+            assert rel_max(cb.f, cf.f) <= 1e-5, (vw, ks.vs.nv, rel_max(cb.f, cf.f))
+            check_macro(cb.prim, co.prim, 1e-4)
+            for d in (alone, f, b):
+                d.close()
+    finally:
+        gpu_ctx.set_sweep_vw(0)
+
+
 def test_residual_fused_matches_oracle(gpu_ctx):
     ks = kb.SolverSet(kb.configs.sod_1d1f1v(nx=64))
     ctr, face = kb.init_fvm(ks)

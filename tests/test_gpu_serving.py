@@ -10,6 +10,13 @@ from conftest import kb
 pytestmark = pytest.mark.gpu
 
 
+@pytest.fixture(autouse=True)
+def _full_width_sweep_after(gpu_ctx):
+    """DeviceBackend sets its velocity cap on the shared context; later tests expect the default."""
+    yield
+    gpu_ctx.set_sweep_vw(0)
+
+
 def _mix():
     inst = kb.configs.batch_instances(4, seed=7)
     reqs = []
