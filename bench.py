@@ -127,7 +127,7 @@ def main():
     ap.add_argument("--serve", type=int, default=48, metavar="N",
                     help="serving leg: N requests of the mixed stream (0 skips it)")
     ap.add_argument("--serve-rate", type=float, default=100.0, help="Poisson arrival rate of the serving leg (1/s)")
-    ap.add_argument("--serve-chunk", type=int, default=100, help="steps per continuous-batching chunk of the serving leg")
+    ap.add_argument("--serve-chunk", type=int, default=400, help="steps per continuous-batching chunk of the serving leg")
     ap.add_argument("--serve-serial", action="store_true", help="serving leg: prepare arrivals between chunks only")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 5 times (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")

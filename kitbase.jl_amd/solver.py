@@ -423,7 +423,7 @@ class Context:
         _lib.check(self.lib.kbg_ctx_set_schedule(self.handle, seg_cells, waves))
 
     def set_graphs(self, enable: bool):
-        """Replay batch steps through HIP graphs (kbg_ctx_set_graphs; default on)."""
+        """Replay batch steps through HIP graphs (kbg_ctx_set_graphs; default off)."""
         _lib.check(self.lib.kbg_ctx_set_graphs(self.handle, 1 if enable else 0))
 
     def set_sweep_steps(self, steps: int):
