@@ -491,15 +491,17 @@ KBG_D void sweep1d(const Solve1D& S, const WorkItem& wi, int par, int lane, int 
             An = T(pn[0] * sqrt(pn[2] / kPi)) * rTi;
             nl2 = T(-pn[2] * kLog2e);
             Un = T(pn[1]);
-            kbn = T(K / (2.0 * pn[2]));
+            kbn = NF == 2 ? T(K / (2.0 * pn[2])) : T(0);  // the b distribution's factor only
             if (SHAK && shak) {
                 const double q = 0.5 * tot[NRAW];
                 pre = T(0.8 * (1.0 - S.Pr) * (po[2] * po[2]) / po[0] * q) * rTi;
-                Ao = T(po[0] * sqrt(po[2] / kPi));
+                // the heat-flux prefactor rides on the old equilibrium's amplitude (pass 2 uses it
+                // only in their product)
+                Ao = T(po[0] * sqrt(po[2] / kPi)) * pre;
                 nlo2 = T(-po[2] * kLog2e);
                 Uo = T(po[1]);
                 lo2 = T(2.0 * po[2]);
-                kbo = T(K / (2.0 * po[2]));
+                kbo = NF == 2 ? T(K / (2.0 * po[2])) : T(0);
             }
             if constexpr (!LAST) {
                 // the first of two steps: the cells it updates carry w, prim to the second
@@ -605,7 +607,7 @@ KBG_D void sweep1d(const Solve1D& S, const WorkItem& wi, int par, int lane, int 
                 T MB = MH * kbn;
                 if constexpr (SHAK) {
                     const T co = ch.u[j] - Uo, c2 = co * co;
-                    const T t = pre * co * Mo[j];
+                    const T t = co * Mo[j];  // Mo carries the prefactor
                     if constexpr (NF == 1) {
                         MH += t * (lo2 * c2 - T(5));
                     } else {
