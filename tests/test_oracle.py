@@ -425,7 +425,7 @@ def test_shakhov_relaxes_heat_flux_at_the_prandtl_rate():
     c = u
     # odd perturbation c (c^2 - 3 / (2 lambda)): a heat flux with no mass, momentum or energy
     h0 = A(H * (1 + 0.05 * (c * c - 3.0 / (2 * prim[2])) * c))
-    b0 = A(h0 * K / (2 * prim[2]))
+    b0 = A(H * K / (2 * prim[2]) * (1 + 0.1 * c))  # with a heat flux of its own: sum c b != 0
     qf = lambda h, b: 0.5 * (np.dot(w, c ** 3 * h) + np.dot(w, c * b))
     ratio = {}
     for coll in (0, 1):
@@ -746,3 +746,202 @@ def test_pspace2d_uniform_geometry():
     assert np.allclose(a4, [[0.5, 0.5, 0.5, 0.5]])
     n4 = normals.reshape(-1, 4, 2)
     assert np.allclose(np.abs(n4).sum(axis=2), 1.0, atol=1e-5)
+
+
+# ---------------------------------------------------------------- Shakhov and 1F3V steps in closed form
+
+def test_shakhov_1f1v_homogeneous_step_closed_form():
+    """Homogeneous 1F1V step (no fluxes) of a gas carrying a heat flux q0, x = dt / tau.
+
+    With the 1F1V Shakhov term S = 0.8 (1 - Pr) lambda^2 / rho c q (2 lambda c^2 - 5) M (the 3D
+    constant 5 in one velocity dimension, theory_equilibrium.jl:178-186), Gaussian moments give
+      int c S = -0.8 (1 - Pr) lambda q      (momentum injected into f, w is left alone)
+      int c^3 S / 2 = 0                      (the heat flux relaxes at the BGK rate)
+    so after one step  sum u f - rho U = -0.8 (1 - Pr) lambda q0 x / (1 + x)  and  q = q0 / (1 + x),
+    while BGK injects no momentum.  A constant 3 in place of 5 would inject none and relax q more
+    slowly; a missing 1 / rho or lambda^2 changes the injected momentum."""
+    u, w = rect_grid(2001, -14.0, 14.0)
+    gam, Pr, mu, dt = 3.0, 2.0 / 3.0, 0.05, 0.01
+    rho, U, lam = 1.3, 0.4, 0.7
+    c = u - U
+    M = maxwellian(u, rho, U, lam)
+    f0 = A(M * (1 + 0.04 * c * (c * c - 3.0 / (2 * lam))))
+    Wm = A([np.dot(w, f0), np.dot(w, u * f0), 0.5 * np.dot(w, u * u * f0)])
+    P = A(np.zeros(3)); call("orc_conserve_prim", Wm, 3, gam, P)
+    assert np.allclose(P, [rho, U, lam], rtol=1e-12)
+    cP = u - P[1]
+    q0 = 0.5 * np.dot(w, cP ** 3 * f0)
+    assert abs(q0) > 1e-3
+    z3, zn = A(np.zeros(3)), A(np.zeros(u.size))
+    out = {}
+    for coll in (0, 1):
+        W, PP, f = A(Wm.copy()), A(P.copy()), A(f0.copy())
+        RES, AVG = A(np.zeros(3)), A(np.zeros(3))
+        call("orc_step_1f1v", W, PP, f, z3, zn, z3, zn, u, w, u.size, gam, mu, 0.81, Pr, 0.1, dt, RES, AVG, coll)
+        assert np.allclose(W, Wm, rtol=0, atol=0) and np.allclose(PP, P, rtol=1e-15)
+        out[coll] = (0.5 * np.dot(w, cP ** 3 * f), np.dot(w, u * f) - P[0] * P[1], np.dot(w, f) - P[0])
+    # x from the VHS law tau = mu (2 lambda^(1 - omega)) / rho (theory_relaxation.jl:28-36)
+    x = dt / (mu * 2 * P[2] ** (1 - 0.81) / P[0])
+    assert out[0][0] / q0 == pytest.approx(1 / (1 + x), rel=1e-10)
+    assert abs(out[0][1]) < 1e-13 and abs(out[0][2]) < 1e-13
+    assert out[1][0] / q0 == pytest.approx(1 / (1 + x), rel=1e-9)
+    assert out[1][1] == pytest.approx(-0.8 * (1 - Pr) * P[2] * q0 * x / (1 + x), rel=1e-9)
+    assert abs(out[1][2]) < 1e-13
+
+
+def test_shakhov_2f2v_relaxes_heat_flux_at_the_prandtl_rate():
+    """Homogeneous 2F2V step (K = 1, the 3D monatomic reduction) of a gas with a heat flux in both
+    directions, reduced energy b included: BGK takes q to q / (1 + x); Shakhov
+    (theory_equilibrium.jl:227-238 with heat_flux theory_moments.jl:567-578) to
+    q (1 + (1 - Pr) x) / (1 + x), componentwise.  Gaussian moments: the Shakhov term carries
+    (1 - Pr) q of heat flux when K = 1 and no momentum."""
+    u, v, w = tensor_grid(121, -11.0, 11.0, 2)
+    K, Pr, mu, dt = 1.0, 2.0 / 3.0, 0.05, 0.01
+    gam = (K + 4) / (K + 2)
+    rho, U, V, lam = 0.9, 0.3, -0.2, 0.8
+    cu, cv = u - U, v - V
+    c2 = cu * cu + cv * cv
+    H = rho * (lam / np.pi) * np.exp(-lam * c2)
+    pert = 1 + 0.05 * cu * (c2 - 2.0 / lam) - 0.03 * cv * (c2 - 2.0 / lam)
+    h0 = A(H * pert)
+    # b carries a heat flux of its own (sum c b), which the b-part of heat_flux must count
+    b0 = A(H * K / (2 * lam) * (1 + 0.1 * cu - 0.07 * cv))
+    Wm = A([np.dot(w, h0), np.dot(w, u * h0), np.dot(w, v * h0),
+            0.5 * (np.dot(w, (u * u + v * v) * h0) + np.dot(w, b0))])
+    P = A(np.zeros(4)); call("orc_conserve_prim", Wm, 4, gam, P)
+
+    def qvec(h, b):
+        du, dv = u - P[1], v - P[2]
+        d2 = du * du + dv * dv
+        return np.array([0.5 * (np.dot(w, du * d2 * h) + np.dot(w, du * b)),
+                         0.5 * (np.dot(w, dv * d2 * h) + np.dot(w, dv * b))])
+
+    q0 = qvec(h0, b0)
+    assert np.all(np.abs(q0) > 1e-3)
+    z4, zn = A(np.zeros(4)), A(np.zeros(u.size))
+    ratio, mom = {}, {}
+    for coll in (0, 1):
+        W, PP, h, b = A(Wm.copy()), A(P.copy()), A(h0.copy()), A(b0.copy())
+        RES, AVG = A(np.zeros(4)), A(np.zeros(4))
+        call("orc_step_2f2v", W, PP, h, b, z4, zn, zn, z4, zn, zn, z4, zn, zn, z4, zn, zn, u, v, w, u.size, K, gam, mu,
+             0.72, Pr, 0.1, dt, RES, AVG, coll)
+        ratio[coll] = qvec(h, b) / q0
+        mom[coll] = np.array([np.dot(w, u * h), np.dot(w, v * h)]) - P[0] * P[1:3]
+    x = dt / (mu * 2 * P[3] ** (1 - 0.72) / P[0])
+    assert np.allclose(ratio[0], 1 / (1 + x), rtol=1e-9)
+    assert np.allclose(ratio[1], (1 + (1 - Pr) * x) / (1 + x), rtol=1e-8)
+    assert np.all(np.abs(mom[1]) < 1e-12)
+
+
+def _max3(u, v, wv, rho, U, V, Wv, lamx, lamyz):
+    """an anisotropic Gaussian: temperature 1 / (2 lamx) along u, 1 / (2 lamyz) along v and w"""
+    return rho * np.sqrt(lamx / np.pi) * (lamyz / np.pi) * np.exp(-lamx * (u - U) ** 2 - lamyz * ((v - V) ** 2 + (wv - Wv) ** 2))
+
+
+def test_step_1f3v_bgk_fixed_point_relaxation_and_free_transport():
+    """1D1F3V BGK step (step_1f.jl:62-115): (i) the Maxwellian rho (lambda / pi)^(3/2) e^(-lambda |c|^2)
+    of the cell's own state is a fixed point; (ii) an anisotropic Gaussian relaxes its directional
+    temperatures at the BGK rate, T_x' - T = (T_x - T) / (1 + x), keeping rho, U, T; (iii) with no
+    collisions, f' = f + (ffL - ffR) / dx and w' = w + (fwL - fwR) / dx; (iv) Shakhov is refused
+    (KitBase's 1F3V Shakhov branch calls shakhov(..., K) with K undefined, step_1f.jl:91)."""
+    u, v, wv, w = tensor_grid(44, -8.0, 8.0, 3)
+    gam, mu, dt, dx = 5.0 / 3.0, 0.02, 0.004, 0.1
+    n = u.size
+    z5, zn = A(np.zeros(5)), A(np.zeros(n))
+
+    def step(f, W, P, fwL=z5, ffL=zn, fwR=z5, ffR=zn, mu_=mu, coll=0):
+        f, W, P = A(f.copy()), A(W.copy()), A(P.copy())
+        RES, AVG = A(np.zeros(5)), A(np.zeros(5))
+        rc = call("orc_step_1f3v", W, P, f, fwL, ffL, fwR, ffR, u, v, wv, w, n, gam, mu_, 0.72, 1.0, dx, dt, RES, AVG, coll)
+        return rc, f, W, P
+
+    def moments(f):
+        return A([np.dot(w, f), np.dot(w, u * f), np.dot(w, v * f), np.dot(w, wv * f),
+                  0.5 * np.dot(w, (u * u + v * v + wv * wv) * f)])
+
+    rho, U, V, Wv, lam = 1.2, 0.3, -0.1, 0.2, 0.7
+    M = A(_max3(u, v, wv, rho, U, V, Wv, lam, lam))
+    W = moments(M)
+    P = A(np.zeros(5)); call("orc_conserve_prim", W, 5, gam, P)
+    assert np.allclose(P, [rho, U, V, Wv, lam], rtol=1e-12)
+    rc, f, W1, P1 = step(M, W, P)
+    assert rc == 0 and np.allclose(f, M, rtol=1e-12, atol=1e-300)
+    # (ii) directional temperatures Tx = 1/(2*0.5), Ty = Tz such that T = 1 / (2 lam)
+    lx = 0.5
+    lyz = 2.0 / (3.0 / lam - 1.0 / lx)
+    f0 = A(_max3(u, v, wv, rho, U, V, Wv, lx, lyz))
+    W0 = moments(f0)
+    P0 = A(np.zeros(5)); call("orc_conserve_prim", W0, 5, gam, P0)
+    assert P0[4] == pytest.approx(lam, rel=1e-10)
+    rc, f, W1, P1 = step(f0, W0, P0)
+    x = dt / (mu * 2 * P0[4] ** (1 - 0.72) / P0[0])
+    cx2 = lambda g: np.dot(w, (u - P0[1]) ** 2 * g) / np.dot(w, g)  # <c_x^2> = 1 / (2 lambda_x)
+    assert cx2(f0) == pytest.approx(1 / (2 * lx), rel=1e-10)
+    assert cx2(f) - 1 / (2 * P0[4]) == pytest.approx((cx2(f0) - 1 / (2 * P0[4])) / (1 + x), rel=1e-9)
+    assert np.allclose(moments(f), W0, rtol=1e-11, atol=1e-13)
+    # (iii) no collisions
+    rng = np.random.default_rng(29)
+    ffL, ffR = A(1e-3 * rng.standard_normal(n)), A(1e-3 * rng.standard_normal(n))
+    fwL, fwR = A(1e-3 * rng.standard_normal(5)), A(1e-3 * rng.standard_normal(5))
+    rc, f, W1, P1 = step(M, W, P, fwL, ffL, fwR, ffR, mu_=1e9)
+    assert np.allclose(f, M + (ffL - ffR) / dx, rtol=1e-9, atol=1e-15)
+    assert np.allclose(W1, W + (fwL - fwR) / dx, rtol=1e-14)
+    # (iv)
+    rc, *_ = step(M, W, P, coll=1)
+    assert rc != 0
+
+
+def _aniso_grid(nu, nv, a, b):
+    """u: nu midpoints on [-a, a] (fine: the half-range terms jump at u = 0); v: nv on [-b, b]"""
+    u1, w1 = rect_grid(nu, -a, a)
+    v1, wv1 = rect_grid(nv, -b, b)
+    U, V = np.meshgrid(u1, v1, indexing="xy")  # u fastest
+    return A(U.ravel()), A(V.ravel()), A(np.outer(wv1, w1).ravel())
+
+
+def _moments_2f2v(u, v, w, fh, fb):
+    return np.array([np.dot(w, fh), np.dot(w, u * fh), np.dot(w, v * fh),
+                     0.5 * (np.dot(w, (u * u + v * v) * fh) + np.dot(w, fb))])
+
+
+@pytest.mark.parametrize("mu", [1e-3, 3e-2, 1.0])
+def test_ugks_2d_analytic_face_moments_match_its_distribution_flux(mu):
+    """2D2F2V UGKS (flux_ugks.jl:121-253) with distinct left/right states and non-zero distribution
+    slopes: fw comes from Gaussian moments (equilibrium, its spatial slopes aL/aR and time slope aT)
+    plus discrete sums of the free-transport part, fh and fb velocity by velocity; on a fine grid the
+    moments of len * (fh, fb) reproduce fw for any tau."""
+    u, v, w = _aniso_grid(1201, 81, 13.0, 10.0)
+    K = 1.0
+    gam = (K + 4) / (K + 2)
+    pl, pr = [1.0, 0.25, 0.1, 0.8], [0.7, -0.15, -0.2, 1.1]
+    hL, bL = _two_f_2v(pl, u, v, K)
+    hR, bR = _two_f_2v(pr, u, v, K)
+    WL, WR = A(np.zeros(4)), A(np.zeros(4))
+    call("orc_prim_conserve", A(pl), 4, gam, WL); call("orc_prim_conserve", A(pr), 4, gam, WR)
+    shL, sbL = A(0.3 * (1 + 0.2 * u - 0.1 * v) * hL), A(0.3 * (1 + 0.2 * u - 0.1 * v) * bL)
+    shR, sbR = A(-0.2 * (1 - 0.1 * u + 0.3 * v) * hR), A(-0.2 * (1 - 0.1 * u + 0.3 * v) * bR)
+    n, length = u.size, 0.6
+    fw = A(np.zeros(4)); fh = A(np.zeros(n)); fb = A(np.zeros(n))
+    call("orc_flux_ugks_2f2v", fw, fh, fb, WL, hL, bL, WR, hR, bR, u, v, w, n, K, gam, mu, 0.72, 2.0 / 3.0, 2e-3, 0.04,
+         0.06, length, shL, sbL, shR, sbR)
+    mom = _moments_2f2v(u, v, w, fh, fb)
+    assert np.allclose(mom, fw, rtol=1e-6, atol=1e-8 * np.max(np.abs(fw)))
+
+
+@pytest.mark.parametrize("mu", [1e-3, 1e-1])
+def test_kcu_2d_analytic_face_moments_match_its_distribution_flux(mu):
+    """2D2F2V KCU (flux_kcu.jl:898-967) with distinct left/right states: the moments of
+    len * (fh, fb) reproduce the analytic fw."""
+    u, v, w = _aniso_grid(1201, 81, 13.0, 10.0)
+    K = 1.0
+    gam = (K + 4) / (K + 2)
+    pl, pr = [1.0, 0.25, 0.1, 0.8], [0.7, -0.15, -0.2, 1.1]
+    hL, bL = _two_f_2v(pl, u, v, K)
+    hR, bR = _two_f_2v(pr, u, v, K)
+    WL, WR = A(np.zeros(4)), A(np.zeros(4))
+    call("orc_prim_conserve", A(pl), 4, gam, WL); call("orc_prim_conserve", A(pr), 4, gam, WR)
+    n, length = u.size, 0.6
+    fw = A(np.zeros(4)); fh = A(np.zeros(n)); fb = A(np.zeros(n))
+    call("orc_flux_kcu_2f2v", fw, fh, fb, WL, hL, bL, WR, hR, bR, u, v, w, n, K, gam, mu, 0.72, 1.0, 2e-3, length)
+    mom = _moments_2f2v(u, v, w, fh, fb)
+    assert np.allclose(mom, fw, rtol=1e-8, atol=1e-12)
