@@ -3,11 +3,25 @@
 
 Julia glue for libkbgpu.so: device-resident KitBase solves stepped on an MI355X through the C ABI of
 `include/kbgpu.h`. A `DeviceSolve` replaces KitBase's `ctr` / `face` arrays in the calls to
-`reconstruct!`, `evolve!`, `update!` and `solve!`; `download!` writes the state back into the host
-`ControlVolume` / `Interface` arrays (for `extract_sol`, `write_jld`, plotting).
+`reconstruct!`, `evolve!`, `update!`, `timestep` and `solve!`; `download!` writes the state back into
+the host `ControlVolume` / `Interface` arrays (for `extract_sol`, `write_jld`, plotting).
 
-Supported: 1D1F1V, 1D2F1V and 1D1F3V (`Interface1F/2F`, `ControlVolume1F/2F`), 2D2F2V with
-`a1face`/`a2face`; fluxes :kfvs, :kcu, :ugks; collisions :bgk, :shakhov; boundaries :fix, :maxwell.
+Supported: 1D1F1V, 1D2F1V and 1D1F3V (`ControlVolume1F/2F` over `eachindex(ps.x)`, ghosts included;
+`Interface1F/2F` faces 1..nx+1), and 2D2F2V without ghost cells (`ControlVolume2F` matrices with
+`a1face` (nx+1) x ny and `a2face` nx x (ny+1)); fluxes :kfvs, :kcu, :ugks; collisions :bgk,
+:shakhov (not 1D1F3V, whose KitBase step! cannot run it, step_1f.jl:91); boundaries :fix, :maxwell.
+
+KitBase v0.9.31 fields read here (file:line of the struct that defines each):
+- `KS.set.space, flux, collision, limiter, interpOrder, boundary, cfl, maxTime`  Setup, struct_setup.jl:106-119
+  (`boundary` is a vector of 2 (1D) or 4 (2D) strings, struct_setup.jl:134-142)
+- `KS.gas.K, γ, μᵣ, ω, Pr`                                                    Gas, struct_property.jl:51-64
+- `KS.vs.u, v, w, weights, u1, v1`                         VSpace1D/2D/3D, phase_velocity.jl:14-21, 128-140, 341-358
+- `KS.ps.x0, x1, nx, x, dx` (1D)                                                PSpace1D, geo_struct.jl:10-16
+- `KS.ps.nx, ny, x, dx, dy, vertices, areas, n` (2D)                            PSpace2D, geo_struct.jl:47-61
+- `KS.ib.bc, KS.ib.p`: `bc(x, p)` in 1D (cfg_sod.jl:32-38), `bc(x, y, p)` in 2D (cfg_cavity.jl:29-36),
+  called as evolve_boundary.jl:19,35 and :77,100,123,147 call them            IB1F/IB2F, struct_ib.jl:33-61
+- `ctr[i].w, prim, f` / `h, b` (and `sw, sf` / `sh, sb` for slopes)      ControlVolume1F/2F, struct_ctr.jl:51-58, 94-103
+- `face[k].fw, ff` / `fh, fb`                                             Interface1F/2F, struct_face.jl:37-40, 56-60
 """
 module KitBaseAMD
 
@@ -40,6 +54,8 @@ function check(rc::Integer)
     error("kbgpu error $rc: $msg")
 end
 
+lookup(table, key, what) = haskey(table, Symbol(key)) ? table[Symbol(key)] : error("$what $(key) is not supported by libkbgpu")
+
 mutable struct DeviceContext
     handle::Ptr{Cvoid}
     function DeviceContext(device::Integer=0)
@@ -58,55 +74,165 @@ mutable struct DeviceSolve
     handle::Ptr{Cvoid}
     ctx::DeviceContext
     ks::SolverSet
+    dim::Int
     keep::Vector{Any}
 end
 
-# host arrays in KitBase's own layout: velocity-contiguous per cell, ghost cells included
-flat(x) = collect(Float64, vec(x))
+# host arrays in KitBase's own layout: column-major, velocity-contiguous per cell, ghost cells included
+flat(x) = collect(Float64, vec(parent(x)))
 
-function DeviceSolve(ks::SolverSet, ctr, face; precision=:fp32, ctx=DeviceContext(0))
-    keep = Any[]
+# the four boundary symbols of the descriptor: Setup.boundary holds 2 (1D) or 4 (2D) entries
+function boundary_codes(set)
+    b = set.boundary isa AbstractVector ? set.boundary : [set.boundary]
+    codes = [Cint(lookup(BCS, s, "boundary")) for s in b]
+    length(codes) == 1 && (codes = fill(codes[1], 4))
+    return ntuple(k -> k <= length(codes) ? codes[k] : Cint(0), 4)
+end
+
+"""
+Per-face [n1, n2, len] of the faces the 2D evolve! visits, in the library's face order
+(x-faces (i, j) at (i-1) + (nx+1)(j-1), y-faces (i, j) at (i-1) + nx(j-1)):
+interior faces take the normal and edge length of the cell on their low side
+(solver_evolution.jl:645-646, 663-664); walls take the outward normal of the boundary cell and its
+dy / dx (evolve_boundary.jl:72-152).
+"""
+function face_geometry_2d(ps)
+    nx, ny = ps.nx, ps.ny
+    xg = zeros(3, nx + 1, ny)
+    yg = zeros(3, nx, ny + 1)
+    for j in 1:ny
+        for i in 2:nx
+            xg[1:2, i, j] .= ps.n[i-1, j, 2]
+            xg[3, i, j] = ps.areas[i-1, j, 2]
+        end
+        xg[1:2, 1, j] .= -ps.n[1, j, 4]
+        xg[3, 1, j] = ps.dy[1, j]
+        xg[1:2, nx+1, j] .= ps.n[nx, j, 2]
+        xg[3, nx+1, j] = ps.dy[nx, j]
+    end
+    for i in 1:nx
+        for j in 2:ny
+            yg[1:2, i, j] .= ps.n[i, j-1, 3]
+            yg[3, i, j] = ps.areas[i, j-1, 3]
+        end
+        yg[1:2, i, 1] .= -ps.n[i, 1, 1]
+        yg[3, i, 1] = ps.dx[i, 1]
+        yg[1:2, i, ny+1] .= ps.n[i, ny, 3]
+        yg[3, i, ny+1] = ps.dx[i, ny]
+    end
+    return xg, yg
+end
+
+"""
+Global-frame wall primitive vectors of the boundary faces, from `KS.ib.bc(xc, yc, KS.ib.p)` at the
+edge midpoints, as evolve_boundary.jl:74-76, 97-99, 120-122, 144-146 evaluate them (the library
+rotates them into each face's frame, as `local_frame` does there).
+"""
+function wall_prims_2d(KS)
+    ps = KS.ps
+    nx, ny = ps.nx, ps.ny
+    vt = ps.vertices
+    xw = zeros(4, nx + 1, ny)
+    yw = zeros(4, nx, ny + 1)
+    mid(i, j, a, b) = ((vt[i, j, a, 1] + vt[i, j, b, 1]) / 2, (vt[i, j, a, 2] + vt[i, j, b, 2]) / 2)
+    for j in 1:ny
+        xw[:, 1, j] .= KS.ib.bc(mid(1, j, 1, 4)..., KS.ib.p)
+        xw[:, nx+1, j] .= KS.ib.bc(mid(nx, j, 2, 3)..., KS.ib.p)
+    end
+    for i in 1:nx
+        yw[:, i, 1] .= KS.ib.bc(mid(i, 1, 1, 2)..., KS.ib.p)
+        yw[:, i, ny+1] .= KS.ib.bc(mid(i, ny, 3, 4)..., KS.ib.p)
+    end
+    return xw, yw
+end
+
+function make_desc(ks::SolverSet, precision, keep)
     ptr(a) = (a = flat(a); push!(keep, a); pointer(a))
     ps, vs, gas = ks.ps, ks.vs, ks.gas
-    dim = parse(Int, ks.set.space[1:1])
-    nf = ks.set.space[3] == '2' ? 2 : 1
-    vdim = parse(Int, ks.set.space[end-1:end-1])
-    bcs = ks.set.boundary isa Symbol || ks.set.boundary isa AbstractString ? fill(Symbol(ks.set.boundary), 4) : Symbol.(ks.set.boundary)
+    space = ks.set.space
+    dim = parse(Int, space[1:1])
+    nf = space[3] == '2' ? 2 : 1
+    vdim = parse(Int, space[end-1:end-1])
+    nul = Ptr{Cdouble}(C_NULL)
     u = ptr(vs.u); w = ptr(vs.weights)
-    v = vdim >= 2 ? ptr(vs.v) : Ptr{Cdouble}(C_NULL)
-    ww = vdim == 3 ? ptr(vs.w) : Ptr{Cdouble}(C_NULL)
+    v = vdim >= 2 ? ptr(vs.v) : nul
+    ww = vdim == 3 ? ptr(vs.w) : nul
+    prec = precision == :fp64 ? Cint(1) : Cint(0)
+    common = (lookup(FLUX, ks.set.flux, "flux"), lookup(COLL, ks.set.collision, "collision"),
+              lookup(LIMITER, ks.set.limiter, "limiter"), ks.set.interpOrder, boundary_codes(ks.set))
     if dim == 1
-        dx = ptr(parent(ps.dx))
-        bcp = ptr(vcat(ks.ib.bc(ps.x0), ks.ib.bc(ps.x1)))
-        d = KbgDesc(1, nf, vdim, length(vs.u), ps.nx, 1, ps.nxg, precision == :fp64 ? 1 : 0,
-                    FLUX[Symbol(ks.set.flux)], COLL[Symbol(ks.set.collision)], LIMITER[Symbol(ks.set.limiter)],
-                    ks.set.interpOrder, Tuple(Cint(BCS[b]) for b in bcs), gas.K, gas.γ, gas.μᵣ, gas.ω, gas.Pr,
-                    u, v, ww, w, dx, C_NULL, bcp, C_NULL, C_NULL, C_NULL, C_NULL)
-    else
-        error("2D: build xgeo/ygeo (face normals, lengths) and the wall states as kitbase.jl_amd/geometry.py does")
+        ng = 1 - first(axes(ps.x, 1))                  # ghost layers of the OffsetArray, update_boundary.jl:40
+        dx = ptr(ps.dx)                                # parent: cells 1-ng .. nx+ng
+        bcp = ptr(vcat(ks.ib.bc(ps.x0, ks.ib.p), ks.ib.bc(ps.x1, ks.ib.p)))
+        return KbgDesc(1, nf, vdim, length(vs.u), ps.nx, 1, ng, prec, common...,
+                       gas.K, gas.γ, gas.μᵣ, gas.ω, gas.Pr, u, v, ww, w, dx, nul, bcp, nul, nul, nul, nul)
+    elseif dim == 2
+        (first(axes(ps.x, 1)) == 1 && first(axes(ps.x, 2)) == 1) ||
+            error("libkbgpu steps 2D solves without ghost cells (PSpace2D with NGX = NGY = 0)")
+        xg, yg = face_geometry_2d(ps)
+        xw, yw = wall_prims_2d(ks)
+        return KbgDesc(2, nf, vdim, length(vs.u), ps.nx, ps.ny, 0, prec, common...,
+                       gas.K, gas.γ, gas.μᵣ, gas.ω, gas.Pr, u, v, ww, w, ptr(ps.dx), ptr(ps.dy), nul,
+                       ptr(xg), ptr(yg), ptr(xw), ptr(yw))
     end
+    error("space $(space) is not supported by libkbgpu")
+end
+
+function create(ks::SolverSet, precision, ctx)
+    keep = Any[]
+    d = make_desc(ks, precision, keep)
     h = Ref{Ptr{Cvoid}}(C_NULL)
-    GC.@preserve keep check(ccall((:kbg_solve_create, libkbgpu), Cint, (Ptr{Cvoid}, Ref{KbgDesc}, Ref{Ptr{Cvoid}}), ctx.handle, Ref(d), h))
-    dev = DeviceSolve(h[], ctx, ks, keep)
+    GC.@preserve keep check(ccall((:kbg_solve_create, libkbgpu), Cint, (Ptr{Cvoid}, Ref{KbgDesc}, Ref{Ptr{Cvoid}}),
+                                  ctx.handle, Ref(d), h))
+    dev = DeviceSolve(h[], ctx, ks, d.dim, keep)
     finalizer(s -> ccall((:kbg_solve_destroy, libkbgpu), Cint, (Ptr{Cvoid},), s.handle), dev)
-    upload!(dev, ctr, face)
     return dev
+end
+
+"""
+    DeviceSolve(ks, ctr, face; precision=:fp32, ctx)            # 1D
+    DeviceSolve(ks, ctr, a1face, a2face; precision=:fp32, ctx)  # 2D
+
+Upload the state of `init_fvm` (or of a checkpoint) once; it stays resident in HBM.
+"""
+function DeviceSolve(ks::SolverSet, ctr, face; precision=:fp32, ctx=DeviceContext(0))
+    dev = create(ks, precision, ctx)
+    dev.dim == 1 || error("2D solves take DeviceSolve(ks, ctr, a1face, a2face)")
+    return upload!(dev, ctr, face)
+end
+
+function DeviceSolve(ks::SolverSet, ctr, a1face, a2face; precision=:fp32, ctx=DeviceContext(0))
+    dev = create(ks, precision, ctx)
+    dev.dim == 2 || error("1D solves take DeviceSolve(ks, ctr, face)")
+    return upload!(dev, ctr, a1face, a2face)
 end
 
 cellvec(ctr, field) = reduce(vcat, [flat(getfield(c, field)) for c in ctr])
 cellf(ctr) = reduce(vcat, [hasproperty(c, :h) ? vcat(flat(c.h), flat(c.b)) : flat(c.f) for c in ctr])
-facevec(face, field) = reduce(vcat, [flat(getfield(f, field)) for f in face])
+facevec(face) = reduce(vcat, [flat(f.fw) for f in face])
 faceff(face) = reduce(vcat, [hasproperty(f, :fh) ? vcat(flat(f.fh), flat(f.fb)) : flat(f.ff) for f in face])
 
-function upload!(dev::DeviceSolve, ctr, face=nothing)
+function upload_cells!(dev, ctr)
     w, prim, f = cellvec(ctr, :w), cellvec(ctr, :prim), cellf(ctr)
     GC.@preserve w prim f check(ccall((:kbg_upload, libkbgpu), Cint, (Ptr{Cvoid}, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}),
                                       dev.handle, w, prim, f))
-    if face !== nothing
-        fw, ff = facevec(face, :fw), faceff(face)
-        GC.@preserve fw ff check(ccall((:kbg_upload_faces, libkbgpu), Cint, (Ptr{Cvoid}, Ptr{Cdouble}, Ptr{Cdouble}),
-                                       dev.handle, fw, ff))
-    end
+end
+
+function upload_faces!(dev, fw, ff)
+    GC.@preserve fw ff check(ccall((:kbg_upload_faces, libkbgpu), Cint, (Ptr{Cvoid}, Ptr{Cdouble}, Ptr{Cdouble}),
+                                   dev.handle, fw, ff))
+end
+
+function upload!(dev::DeviceSolve, ctr, face=nothing)
+    upload_cells!(dev, ctr)
+    face === nothing || upload_faces!(dev, facevec(face), faceff(face))
+    return dev
+end
+
+# 2D: x-faces (a1face, column-major) then y-faces (a2face)
+function upload!(dev::DeviceSolve, ctr, a1face, a2face)
+    upload_cells!(dev, ctr)
+    upload_faces!(dev, vcat(facevec(a1face), facevec(a2face)), vcat(faceff(a1face), faceff(a2face)))
     return dev
 end
 
@@ -130,23 +256,35 @@ function download!(ctr, dev::DeviceSolve)
     return ctr
 end
 
-# KitBase's entry points on device-resident state (solver_reconstruction.jl:56, solver_evolution.jl:39,
-# solver_update.jl:61, solver_main.jl:12). `mode` / `coll` / `bc` must match the SolverSet the solve
+# KitBase's entry points on device-resident state (solver_reconstruction.jl:56, solver_evolution.jl:39,608,
+# solver_update.jl:61,193, solver_main.jl:12,90). `mode` / `coll` / `bc` must match the SolverSet the solve
 # was created from: the device solve carries them.
 KitBase.reconstruct!(KS::SolverSet, dev::DeviceSolve) = check(ccall((:kbg_reconstruct, libkbgpu), Cint, (Ptr{Cvoid},), dev.handle))
 
-function KitBase.evolve!(KS::SolverSet, dev::DeviceSolve, face, dt; mode=Symbol(KS.set.flux), bc=nothing)
-    mode == Symbol(KS.set.flux) || error("mode $mode differs from the device solve's flux $(KS.set.flux)")
+function check_mode(KS, mode, coll)
+    mode === nothing || Symbol(mode) == Symbol(KS.set.flux) || error("mode $mode differs from the device solve's flux $(KS.set.flux)")
+    coll === nothing || Symbol(coll) == Symbol(KS.set.collision) ||
+        error("coll $coll differs from the device solve's collision $(KS.set.collision)")
+end
+
+function KitBase.evolve!(KS::SolverSet, dev::DeviceSolve, face, dt; mode=nothing, bc=nothing)
+    check_mode(KS, mode, nothing)
     check(ccall((:kbg_evolve, libkbgpu), Cint, (Ptr{Cvoid}, Cdouble), dev.handle, dt))
 end
 
-function KitBase.update!(KS::SolverSet, dev::DeviceSolve, face, dt, residual; coll=Symbol(KS.set.collision), bc=nothing)
-    coll == Symbol(KS.set.collision) || error("coll $coll differs from the device solve's collision $(KS.set.collision)")
+KitBase.evolve!(KS::SolverSet, dev::DeviceSolve, a1face, a2face, dt; mode=nothing, bc=nothing) =
+    KitBase.evolve!(KS, dev, nothing, dt; mode=mode, bc=bc)
+
+function KitBase.update!(KS::SolverSet, dev::DeviceSolve, face, dt, residual; coll=nothing, bc=nothing, kwargs...)
+    check_mode(KS, nothing, coll)
     r = zeros(Cdouble, 5)
     check(ccall((:kbg_update, libkbgpu), Cint, (Ptr{Cvoid}, Cdouble, Ptr{Cdouble}), dev.handle, dt, r))
     residual .= r[1:length(residual)]
     return nothing
 end
+
+KitBase.update!(KS::SolverSet, dev::DeviceSolve, a1face, a2face, dt, residual; coll=nothing, bc=nothing, kwargs...) =
+    KitBase.update!(KS, dev, nothing, dt, residual; coll=coll, bc=bc)
 
 # timestep(KS, ctr, simTime) (solver_main.jl:240-342) from the device-resident prim
 function KitBase.timestep(KS::SolverSet, dev::DeviceSolve, simTime=0.0)
@@ -166,6 +304,20 @@ function step!(dev::DeviceSolve, dt, nsteps::Integer)
     r = zeros(Cdouble, 5)
     check(ccall((:kbg_step, libkbgpu), Cint, (Ptr{Cvoid}, Cdouble, Cint, Ptr{Cdouble}), dev.handle, dt, nsteps, r))
     return r
+end
+
+"""
+    solve!(KS, dev, simTime) -> simTime + nt dt
+
+`solve!`'s loop (solver_main.jl:12-76, 90-150) on the device: dt from `timestep` once, `nt =
+floor(maxTime / dt) + 1` fused steps. The JLD2 checkpoints of the reference loop are the caller's:
+`download!(ctr, dev)` then `write_jld(KS, ctr, t)`.
+"""
+function KitBase.solve!(KS::SolverSet, dev::DeviceSolve, simTime; steady=false)
+    dt = KitBase.timestep(KS, dev, simTime)
+    nt = Int(floor(KS.set.maxTime / dt)) + 1
+    step!(dev, dt, nt)
+    return simTime + nt * dt
 end
 
 """
