@@ -8,16 +8,23 @@ by kbg_batch_step.  One "step" advances every instance by one time step (its own
 
 Printed (one JSON line on stdout):
   value          cell x velocity updates/s of the FP32 batched path (wall clock around the K steps)
-  fp64_plain     the same instances stepped one solve at a time in FP64 (kbg_step per solve)
-  l2_vs_fp64     max over instances of the relative L2 error in rho, U, T of the FP32 batch
-                 against the FP64 single-solve results after the same steps
+  l2_vs_fp64     accuracy of the FP32 batch at the workload's end time t = 0.2 (nt = floor(0.2 / dt) + 1
+                 steps per instance, up to ~5500): plain relative L2 and L-inf errors in rho, U, T of
+                 every instance against the FP64 single-solve GPU path, the largest over instances,
+                 and the instances above the 1e-4 L2 bar (none expected)
+  fp64_plain     the FP64 single-solve path's updates/s (kbg_step per solve, one at a time) over the
+                 same runs to t = 0.2
   roofline       HBM roofline of the sweep kernels: algorithmic bytes per step (8 B per
                  cell x velocity + 48 B per cell) / device time per step (HIP events on the
                  library's stream around the step loop); the sweep advances two steps per pass
                  over f, so its measured HBM traffic ("traffic", per step) is below that figure
-  cpu_baseline   the FP64 CPU oracle (OpenMP) on a bounded sample of the same instances
+  cpu_baseline   the FP64 CPU oracle (OpenMP) on a bounded sample of the same instances run to
+                 t = 0.2; the same instances' GPU results (FP32 batch, FP64 path) are checked against
+                 its output in "accuracy_vs_oracle"
+  serving        the serving leg: Poisson arrivals at each rate of --serve-rates, mixing the batch's
+                 tubes, 2D cavities 32^2-128^2 x 16^2-48^2 and 1D1F3V 24^3 requests (p50 / p99)
 
-    python bench.py [--steps K] [--warmup W] [--instances N] [--no-cpu] [--no-fp64]
+    python bench.py [--steps K] [--warmup W] [--instances N] [--no-cpu] [--no-fp64] [--serve N]
 """
 from __future__ import annotations
 
@@ -36,6 +43,7 @@ sys.path.insert(0, str(ROOT / "kitbase.jl_amd"))
 
 HBM_PEAK_GBS = 8000.0
 PROFILES = ROOT / "profiles"
+TOL = 1e-4
 
 
 def log(*a):
@@ -44,12 +52,34 @@ def log(*a):
 
 def macro(prim):
     p = np.asarray(prim)
-    return p[:, 0], p[:, 1], 1.0 / (2.0 * p[:, 2])
+    return {"rho": p[:, 0], "U": p[:, 1], "T": 1.0 / (2.0 * p[:, 2])}
 
 
-def rel_l2(a, b, scale=None):
-    d = np.linalg.norm(np.asarray(a) - np.asarray(b))
-    return float(d / (scale if scale is not None else max(np.linalg.norm(b), 1e-300)))
+def errors(pa, pb):
+    """plain relative L2 and L-inf errors of a against b in rho, U, T"""
+    out = {}
+    ma, mb = macro(pa), macro(pb)
+    for k in ma:
+        d = ma[k] - mb[k]
+        out[k + "_l2"] = float(np.linalg.norm(d) / max(np.linalg.norm(mb[k]), 1e-300))
+        out[k + "_linf"] = float(np.max(np.abs(d)) / max(np.max(np.abs(mb[k])), 1e-300))
+    return out
+
+
+def cpu_info():
+    model = None
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"cpu_model": model, "nproc": os.cpu_count(), "usable_cpus": usable}
 
 
 def build_instances(kb, n):
@@ -63,29 +93,57 @@ def build_instances(kb, n):
     return out
 
 
-def traffic_from_profiles(points):
+def t_end_steps(inst):
+    """KitBase's solve!: nt = floor(maxTime / dt) + 1 (solver_main.jl:28)"""
+    return np.array([int(np.floor(ks.set.maxTime / dt)) + 1 for ks, _, _, dt in inst])
+
+
+def profile_traffic(points, schedule):
     """HBM bytes per step of the sweep kernels from the committed rocprofv3 PMC summary
     (FETCH_SIZE + WRITE_SIZE per step, scaled by the copy-kernel calibration), when it was taken on
-    this same workload (same number of cell x velocity points)."""
+    this same workload and schedule; returns (bytes, schedule it was taken under) or (None, None)."""
     p = PROFILES / "sweep1d_fp32_batch512_pmc.txt"
     if not p.exists():
-        return None
+        return None, None
     vals = {}
     for line in p.read_text().splitlines():
-        if ":" in line and line.split(":", 1)[0].strip() in ("points_per_step", "hbm_bytes_per_step"):
+        if ":" in line:
             k, v = line.split(":", 1)
-            vals[k.strip()] = float(v.split()[0])
-    if vals.get("points_per_step") != float(points) or "hbm_bytes_per_step" not in vals:
-        return None
-    return vals["hbm_bytes_per_step"]
+            vals[k.strip()] = v.strip()
+    try:
+        if float(vals.get("points_per_step", "nan").split()[0]) != float(points):
+            return None, None
+        traffic = float(vals["hbm_bytes_per_step"].split()[0])
+    except (KeyError, ValueError):
+        return None, None
+    return traffic, vals.get("schedule")
+
+
+def run_batch_to(kb, ctx, inst, nts, precision="fp32"):
+    """Fresh uploads of every instance; all advance together, each leaving at its own step count."""
+    devs = [kb.DeviceSolve(ks, ctr, face, precision=precision, ctx=ctx) for ks, ctr, face, _ in inst]
+    done, live = 0, list(np.argsort(nts, kind="stable"))
+    t0 = time.perf_counter()
+    while live:
+        m = int(min(nts[i] for i in live))
+        kb.batch_step([devs[i] for i in live], [inst[i][3] for i in live], m - done, ctx=ctx)
+        done = m
+        live = [i for i in live if nts[i] > done]
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    out = []
+    for d, (ks, ctr, face, _) in zip(devs, inst):
+        c = copy.deepcopy(ctr)
+        d.download(c)
+        d.close()
+        out.append(c.prim.copy())
+    return out, wall
 
 
 def cavity_leg(kb, ctx, steps):
     """Config 3 geometry (64^2 cells x 32^2 Maxwell-quadrature velocities, UGKS + BGK, Maxwell walls)
     stepped by the fused 2D step in FP32 and FP64; device time from the library's step events."""
-    cavity = {"config": "2D2F2V cavity 64^2 x 32^2, UGKS + BGK, Kn 0.075, Maxwell walls", "steps": steps,
-              "kernels": "k_face2d (FP64 face chain) + k_flux2d (face fluxes, inline slopes) + k_cell2d (update)",
-              "bound": "valu: the UGKS face flux is about 100 FP32 operations per face and velocity"}
+    cavity = {"config": "2D2F2V cavity 64^2 x 32^2, UGKS + BGK, Kn 0.075, Maxwell walls", "steps": steps}
     for prec in ("fp32", "fp64"):
         ks = kb.SolverSet(kb.configs.cavity_2d2f2v())
         ctr, face = kb.init_fvm(ks)
@@ -109,6 +167,23 @@ def cavity_leg(kb, ctx, steps):
     return cavity
 
 
+def serving_leg(kb, ctx, n, rates, chunk, overlap):
+    out = {"mix": "75% 1D tubes of the batch (to t = 0.2), 15% lid-driven cavities {32,48,64,96,128}^2 cells x "
+                  "{16,24,32,48}^2 velocities (100 steps), 10% 1D1F3V 24 cells x 24^3 velocities (40 steps); FP32",
+           "requests_per_rate": n, "budget_bytes": 32 << 30, "chunk_steps": chunk, "overlap": overlap, "rates": {}}
+    for rate in rates:
+        reqs = [kb.serving.Request(rid=k, cfg=cfg, nsteps=ns, arrival=t)
+                for k, (kind, cfg, ns, t) in enumerate(kb.configs.serving_requests(n, rate))]
+        srv = kb.serving.Server(kb.serving.DeviceBackend(kb, ctx=ctx), budget_bytes=32 << 30, chunk_steps=chunk,
+                                overlap=overlap)
+        st = srv.run(reqs).summary()
+        out["rates"][str(rate)] = st
+        log(f"[bench] serving {rate:g}/s: {st['requests']} requests, p50 {st['p50_ms']:.1f} ms, p99 {st['p99_ms']:.1f} ms, "
+            f"{st['updates_per_s']:.3e} updates/s, max live {st['max_live']}")
+    ctx.set_sweep_vw(0)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -120,13 +195,13 @@ def main():
     ap.add_argument("--sweep-steps", type=int, default=2, choices=(1, 2),
                     help="time steps per fused sweep launch (2: two per pass over f where the solves allow it)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
-    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 single-solve comparison")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU work of the baseline sample")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the accuracy leg (FP32 and FP64 runs to t = 0.2)")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0, help="target CPU work of the baseline sample")
     ap.add_argument("--cavity-steps", type=int, default=20, help="timed steps of the 2D cavity leg (0 skips it)")
     ap.add_argument("--cavity-only", action="store_true", help="run only the 2D cavity leg (profiling)")
     ap.add_argument("--serve", type=int, default=48, metavar="N",
-                    help="serving leg: N requests of the mixed stream (0 skips it)")
-    ap.add_argument("--serve-rate", type=float, default=100.0, help="Poisson arrival rate of the serving leg (1/s)")
+                    help="serving leg: N requests of the mixed stream per arrival rate (0 skips it)")
+    ap.add_argument("--serve-rates", default="10,20,50,100,200,500", help="Poisson arrival rates of the serving leg (1/s)")
     ap.add_argument("--serve-chunk", type=int, default=400, help="steps per continuous-batching chunk of the serving leg")
     ap.add_argument("--serve-serial", action="store_true", help="serving leg: prepare arrivals between chunks only")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -175,7 +250,7 @@ def main():
     ctx.set_sweep_steps(a.sweep_steps)
     dts = np.array([d for *_, d in inst])
 
-    # ---- FP32 batched path
+    # ---- FP32 batched path: the timed region
     t0 = time.time()
     devs = [kb.DeviceSolve(ks, ctr, face, precision="fp32", ctx=ctx) for ks, ctr, face, _ in inst]
     log(f"[bench] fp32 upload {time.time() - t0:.1f}s")
@@ -183,16 +258,13 @@ def main():
     if a.seg:
         ctx.set_schedule(a.seg, 0)
     elif not a.no_tune:
-        # the tuner's trial steps are warm-up steps: they count toward the steps compared with FP64
         best, trials, taken = kb.autotune.tune(ctx, devs, dts)
-        tuned = {"best": {"seg_cells": best[0], "streams": best[1]},
+        tuned = {"best": {"seg_cells": best[0], "streams": best[1]}, "tuner_steps": taken,
                  "ms_per_step": {f"seg{k[0]}_streams{k[1]}": round(v, 4) for k, v in trials.items()}}
         log(f"[bench] autotuner: seg_cells {best[0]}, streams {best[1]} ({taken} trial steps): "
             + ", ".join(f"{k}: {v:.3f} ms" for k, v in trials.items()))
     if W:
         kb.batch_step(devs, dts, W, ctx=ctx)
-    W += taken
-    log(f"[bench] warmup {W} steps in total")
     ctx.sync()
     t0 = time.perf_counter()
     kb.batch_step(devs, dts, K, ctx=ctx)
@@ -202,56 +274,51 @@ def main():
     value = points * K / wall
     log(f"[bench] fp32 batched: {K} steps in {wall * 1e3:.2f} ms wall ({dev_ms:.2f} ms device), "
         f"{value:.4e} updates/s")
-    res32 = []
-    for d, (ks, ctr, face, _) in zip(devs, inst):
-        c = copy.deepcopy(ctr)
-        d.download(c)
-        res32.append(c.prim.copy())
+    for d in devs:
         d.close()
     devs = None
+    schedule_now = ctx.schedule()
 
-    # ---- FP64 plain single-solve path (same instances, same steps)
-    fp64 = None
-    l2 = None
+    # ---- accuracy at t = 0.2: FP32 batch against the FP64 single-solve path, every instance
+    nts = t_end_steps(inst)
+    res32 = res64 = None
+    fp64 = acc = None
     if not a.no_fp64:
-        tot_wall, tot_dev = 0.0, 0.0
-        worst = [0.0, 0.0, 0.0]
-        worst_case = None
-        worst_u_strict = 0.0
-        for idx, (ks, ctr, face, dt) in enumerate(inst):
+        res32, wall32 = run_batch_to(kb, ctx, inst, nts, "fp32")
+        res64, tot_wall, tot_dev, upd = [], 0.0, 0.0, 0.0
+        for (ks, ctr, face, dt), nt in zip(inst, nts):
             d = kb.DeviceSolve(ks, ctr, face, precision="fp64", ctx=ctx)
-            if W:
-                d.step(dt, W)
             ctx.sync()
             t1 = time.perf_counter()
-            d.step(dt, K)
+            d.step(dt, int(nt))
             tot_wall += time.perf_counter() - t1
             tot_dev += ctx.last_steps_ms()
+            upd += float(ks.ps.nx * ks.vs.nv) * nt
             c = copy.deepcopy(ctr)
             d.download(c)
             d.close()
-            r64, u64, T64 = macro(c.prim)
-            r32, u32, T32 = macro(res32[idx])
-            # velocity errors relative to the characteristic speed |U| + c (the CFL speed of
-            # solver_main.jl:240-288); ||dU|| / ||U|| alone is ill-conditioned where U ~ 0
-            speed = np.abs(u64) + np.sqrt(0.5 * ks.gas.gamma / c.prim[:, 2])
-            e = (rel_l2(r32, r64), rel_l2(u32, u64, np.linalg.norm(speed)), rel_l2(T32, T64))
-            u_strict = rel_l2(u32, u64, max(np.linalg.norm(u64), 1e-300))
-            if max(e) > max(worst):
-                worst_case = (idx, e, ks.ps.nx, ks.vs.nv, ks.set.collision, ks.gas.Kn,
-                              float(np.max(np.abs(u64))), float(np.max(np.abs(u32 - u64))))
-            worst = [max(x, y) for x, y in zip(worst, e)]
-            worst_u_strict = max(worst_u_strict, u_strict)
-        fp64 = {"value": points * K / tot_wall, "unit": "cell-velocity updates/s",
-                "device_value": points * K / (tot_dev * 1e-3),
-                "path": "kbg_step per solve, FP64 storage, one solve at a time"}
-        l2 = {"rho": worst[0], "U": worst[1], "T": worst[2], "tol": 1e-4,
-              "U_scale": "L2 norm of |U| + sound speed",
-              "U_over_norm_U": worst_u_strict, "pass": max(worst) <= 1e-4}
-        log(f"[bench] fp64 plain: {fp64['value']:.4e} updates/s; ratio {value / fp64['value']:.1f}x; "
-            f"max rel L2 rho {worst[0]:.2e} U {worst[1]:.2e} T {worst[2]:.2e}")
-        log(f"[bench] worst instance {worst_case[0]}: errors {worst_case[1]}, nx {worst_case[2]}, nv {worst_case[3]}, "
-            f"{worst_case[4]}, Kn {worst_case[5]:.3e}, max|U| {worst_case[6]:.3e}, max|dU| {worst_case[7]:.3e}")
+            res64.append(c.prim.copy())
+        fp64 = {"value": upd / tot_wall, "unit": "cell-velocity updates/s", "device_value": upd / (tot_dev * 1e-3),
+                "path": "kbg_step per solve, FP64 storage, one solve at a time, each instance to t = 0.2"}
+        rows = []
+        for k, ((ks, *_), r32, r64, nt) in enumerate(zip(inst, res32, res64, nts)):
+            e = errors(r32, r64)
+            rows.append({"instance": k, "nx": ks.ps.nx, "nv": ks.vs.nv, "Kn": ks.gas.Kn, "collision": ks.set.collision,
+                         "steps": int(nt), "rms_U": float(np.sqrt(np.mean(r64[:, 1] ** 2))), **e})
+        keys = [k for k in rows[0] if k.endswith(("_l2", "_linf"))]
+        worst = {k: max(r[k] for r in rows) for k in keys}
+        fails = [r for r in rows if max(r[k] for k in keys if k.endswith("_l2")) > TOL]
+        acc = {"t_end": 0.2, "steps": [int(nts.min()), int(nts.max())], "tol_l2": TOL,
+               "reference": "FP64 single-solve GPU path (held to the oracle at 1e-12..1e-9 by the GPU tests)",
+               "metric": "per instance ||x32 - x64|| / ||x64|| (L2) and max|x32 - x64| / max|x64| (Linf), x = rho, U, T; "
+                         "the largest over the instances",
+               **{k: v for k, v in worst.items()},
+               "instances_over_tol_l2": [{k: r[k] for k in ("instance", "nx", "nv", "Kn", "rms_U", "U_l2")} for r in fails],
+               "instances_over_1e-4_linf": int(sum(1 for r in rows if max(r[k] for k in keys if k.endswith("_linf")) > TOL)),
+               "pass": not fails}
+        log(f"[bench] accuracy at t = 0.2 (FP32 batch vs FP64): " + ", ".join(f"{k} {v:.2e}" for k, v in worst.items())
+            + f"; {len(fails)} instances over {TOL:g} in L2")
+        log(f"[bench] fp64 plain: {fp64['value']:.4e} updates/s; ratio {value / fp64['value']:.1f}x")
 
     # ---- 2D lid-driven cavity (64^2 cells x 32^2 velocities, UGKS, Maxwell walls), fused step
     cavity = cavity_leg(kb, ctx, a.cavity_steps) if a.cavity_steps else None
@@ -259,48 +326,56 @@ def main():
     # ---- serving leg: Poisson arrivals of 1D tubes, 2D cavities and 1D1F3V requests, served together
     serve = None
     if a.serve:
-        reqs = [kb.serving.Request(rid=k, cfg=cfg, nsteps=ns, arrival=t)
-                for k, (kind, cfg, ns, t) in enumerate(kb.configs.serving_requests(a.serve, a.serve_rate))]
-        srv = kb.serving.Server(kb.serving.DeviceBackend(kb, ctx=ctx), budget_bytes=32 << 30, chunk_steps=a.serve_chunk,
-                                overlap=not a.serve_serial)
-        st = srv.run(reqs)
-        serve = dict(st.summary(), rate_per_s=a.serve_rate, budget_bytes=32 << 30, chunk_steps=a.serve_chunk,
-                     overlap=not a.serve_serial,
-                     mix="75% 1D tubes (to t=0.2), 15% 2D cavities 32^2-48^2 x 16^2-24^2 (100 steps), "
-                         "10% 1D1F3V 24 cells x 24^3 (40 steps); FP32")
-        log(f"[bench] serving: {serve['requests']} requests at {a.serve_rate}/s, p50 {serve['p50_ms']:.1f} ms, "
-            f"p99 {serve['p99_ms']:.1f} ms, {serve['updates_per_s']:.3e} updates/s, max live {serve['max_live']}")
+        rates = [float(x) for x in a.serve_rates.split(",") if x]
+        serve = serving_leg(kb, ctx, a.serve, rates, a.serve_chunk, not a.serve_serial)
 
-    # ---- CPU baseline: FP64 oracle with OpenMP on a bounded sample
-    cpu = None
+    # ---- CPU baseline: FP64 oracle with OpenMP on a bounded sample of the instances, run to t = 0.2
+    cpu, acc_oracle = None, None
     if not a.no_cpu:
         sys.path.insert(0, str(ROOT / "oracle"))
         import oracle
 
         nthreads = oracle.num_threads()
-        sample = inst[: min(64, len(inst))]
-        spts = sum(ks.ps.nx * ks.vs.nv for ks, *_ in sample)
-        runs = [(ks, copy.deepcopy(ctr), copy.deepcopy(face), dt) for ks, ctr, face, dt in sample]
-        solvers = [oracle.OracleSolve(ks, c, f) for ks, c, f, _ in runs]
+        # rate estimate from 3 steps of the first instances, then as many instances (in order) as
+        # fit the time budget at their full step counts
+        probe = [(ks, copy.deepcopy(ctr), copy.deepcopy(face), dt) for ks, ctr, face, dt in inst[:8]]
         t1 = time.perf_counter()
-        for o, (_, _, _, dt) in zip(solvers, runs):
-            o.run(dt, 1)
-        per_step = time.perf_counter() - t1
-        nst = int(max(1, min(1000, a.cpu_seconds / max(per_step, 1e-6))))
+        for ks, c, f, dt in probe:
+            oracle.OracleSolve(ks, c, f).run(dt, 3)
+        rate = 3 * sum(ks.ps.nx * ks.vs.nv for ks, *_ in probe) / (time.perf_counter() - t1)
+        budget, m = a.cpu_seconds * rate, 0
+        while m < len(inst) and sum(inst[j][0].ps.nx * inst[j][0].vs.nv * nts[j] for j in range(m + 1)) <= budget:
+            m += 1
+        m = max(m, 1)
+        sample = [(ks, copy.deepcopy(ctr), copy.deepcopy(face), dt) for ks, ctr, face, dt in inst[:m]]
+        supd = float(sum(ks.ps.nx * ks.vs.nv * nts[j] for j, (ks, *_) in enumerate(sample)))
         t1 = time.perf_counter()
-        for o, (_, _, _, dt) in zip(solvers, runs):
-            o.run(dt, nst)
+        outs = []
+        for j, (ks, c, f, dt) in enumerate(sample):
+            oracle.OracleSolve(ks, c, f).run(dt, int(nts[j]))
+            outs.append(c.prim)
         cpu_wall = time.perf_counter() - t1
-        cpu = {"value": spts * nst / cpu_wall, "unit": "cell-velocity updates/s", "cores": nthreads,
-               "kind": "port",
-               "sample": f"FP64 oracle (C restatement of KitBase's 1D1F1V step, OpenMP over cells) on the "
-                         f"first {len(sample)} of the {len(inst)} instances ({spts} points), {nst} steps each, "
+        cpu = {"value": supd / cpu_wall, "unit": "cell-velocity updates/s", "cores": nthreads, "kind": "port",
+               **cpu_info(),
+               "sample": f"FP64 oracle (C restatement of KitBase's 1D1F1V step, OpenMP over cells, {nthreads} threads) "
+                         f"on the first {m} of the {len(inst)} instances, each to t = 0.2 ({int(supd)} updates), "
                          f"{cpu_wall:.1f} s"}
-        log(f"[bench] cpu oracle: {cpu['value']:.4e} updates/s on {nthreads} threads ({cpu_wall:.1f}s)")
+        log(f"[bench] cpu oracle: {cpu['value']:.4e} updates/s on {nthreads} threads ({cpu_wall:.1f}s, {m} instances)")
+        if res32 is not None:
+            e32 = [errors(res32[j], outs[j]) for j in range(m)]
+            e64 = [errors(res64[j], outs[j]) for j in range(m)]
+            acc_oracle = {"instances": m, "t_end": 0.2,
+                          "fp32_batch": {k: max(e[k] for e in e32) for k in e32[0]},
+                          "fp64_path": {k: max(e[k] for e in e64) for k in e64[0]}}
+            acc_oracle["pass"] = max(v for k, v in acc_oracle["fp32_batch"].items() if k.endswith("_l2")) <= TOL
+            log(f"[bench] against the oracle ({m} instances, t = 0.2): fp32 max L2 "
+                f"{max(v for k, v in acc_oracle['fp32_batch'].items() if k.endswith('_l2')):.2e}, fp64 max L2 "
+                f"{max(v for k, v in acc_oracle['fp64_path'].items() if k.endswith('_l2')):.2e}")
 
     ms_step = dev_ms / K
     achieved = alg_bytes_step / (ms_step * 1e-3) / 1e9
-    traffic = traffic_from_profiles(points)
+    sched = f"seg{schedule_now[0]}_streams{schedule_now[1] or 4}_steps{a.sweep_steps}"
+    traffic, tsched = profile_traffic(points, sched)
     out = {
         "metric": "cell×velocity updates/s, FP32-batched vs FP64 plain @1e-4 L2; p99 solve latency",
         "value": value,
@@ -315,22 +390,26 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "batched 1D shock tubes: 512 instances, nx U{64..2048}, nv U{32..256}, "
-                               "Kn 10^U(-4,1), BGK/Shakhov, 1D1F1V KFVS, FP32 f / FP64 w",
+                               "Kn 10^U(-4,1), BGK/Shakhov, 1D1F1V KFVS, FP32 f (departure from the local Maxwellian) / FP64 w",
                    "instances": len(inst), "seed": kb.configs.BATCH_SEED, "cells": cells,
                    "points_per_step": points, "parallelism": "single GPU"},
         "schedule": tuned,
         "cavity_2d": cavity,
         "fp64_plain": fp64,
         "speedup_vs_fp64_plain": (value / fp64["value"]) if fp64 else None,
-        "l2_vs_fp64": l2,
-        "p99_solve_latency_ms": serve["p99_ms"] if serve else None,
+        "l2_vs_fp64": acc,
+        "accuracy_vs_oracle": acc_oracle,
+        "p99_solve_latency_ms": serve["rates"].get("100.0", {}).get("p99_ms") if serve else None,
         "serving": serve,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic if tsched == sched else None,
+                     "traffic_schedule": tsched,
                      "kernel": f"k_sweep1d<float,1F,KFVS>: {a.sweep_steps} time step(s) per launch, one launch "
-                               f"per stream ({ctx.schedule()[1] or 4} concurrent streams, {ctx.schedule()[0]} cells "
+                               f"per stream ({schedule_now[1] or 4} concurrent streams, {schedule_now[0]} cells "
                                "per segment); achieved = per-update algorithmic bytes x updates / device time",
-                     "steps_per_launch": a.sweep_steps,
+                     "limiter": "VALU issue (the per-velocity arithmetic of two time steps per pass over f), "
+                                "not HBM bandwidth: see profiles/sweep1d_fp32_batch512_pmc.txt",
+                     "schedule": sched, "steps_per_launch": a.sweep_steps,
                      "algorithmic_bytes_per_step": alg_bytes_step, "device_ms_per_step": ms_step},
         "cpu_baseline": cpu,
     }

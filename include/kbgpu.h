@@ -120,6 +120,15 @@ int kbg_solve_destroy(kbg_solve *s);
 /* Device bytes a solve with this description occupies on the fused path (admission control). */
 size_t kbg_solve_bytes(const kbg_desc *desc);
 
+/* Failure detection.  No counterpart in KitBase, where a NaN or a negative temperature propagates
+ * silently: every cell update checks the cell's new primitive variables, and a cell whose values are
+ * not finite (KBG_HEALTH_NONFINITE), whose density is not positive (KBG_HEALTH_RHO) or whose
+ * lambda = 1/(2RT) is not positive (KBG_HEALTH_LAMBDA) raises that bit in the solve's flag word.  The
+ * step completes either way; `flags` receives the word after the solve's queued work, and clear != 0
+ * resets it. */
+enum { KBG_HEALTH_NONFINITE = 1, KBG_HEALTH_RHO = 2, KBG_HEALTH_LAMBDA = 4 };
+int kbg_solve_health(kbg_solve *s, int *flags, int clear);
+
 /* Host <-> device copies of the cell and face state (FP64 on the host, converted on the fly). */
 int kbg_upload(kbg_solve *s, const double *w, const double *prim, const double *f);
 int kbg_download(kbg_solve *s, double *w, double *prim, double *f);

@@ -20,6 +20,7 @@ KBG_BGK, KBG_SHAKHOV = 0, 1
 KBG_BC_FIX, KBG_BC_MAXWELL = 0, 1
 KBG_VANLEER, KBG_MINMOD, KBG_SUPERBEE, KBG_VANALBABA, KBG_LINEAR = 0, 1, 2, 3, 4
 KBG_EV_STEPS_BEGIN, KBG_EV_STEPS_END = 14, 15  # event slots around kbg_step / kbg_batch_step loops
+KBG_HEALTH_NONFINITE, KBG_HEALTH_RHO, KBG_HEALTH_LAMBDA = 1, 2, 4  # kbg_solve_health flag bits
 
 FLUX = {"kfvs": KBG_KFVS, "kcu": KBG_KCU, "ugks": KBG_UGKS}
 COLL = {"bgk": KBG_BGK, "shakhov": KBG_SHAKHOV}
@@ -75,6 +76,7 @@ _PROTOS = {
     "kbg_timestep": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, _dp]),
     "kbg_step": (C.c_int, [C.c_void_p, C.c_double, C.c_int, _dp]),
     "kbg_batch_step": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_int, _dp, C.c_int, _dp]),
+    "kbg_solve_health": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_int]),
 }
 
 EXPORTED = tuple(_PROTOS)

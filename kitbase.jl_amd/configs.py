@@ -55,6 +55,14 @@ def cavity_2d2f2v(**over) -> dict:
 CAVITY_STEPS = 5000
 
 
+def cavity_2d1f2v(**over) -> dict:
+    """The lid-driven cavity with one distribution (2D1F2V: no internal degrees of freedom, gamma = 2),
+    KFVS + BGK (KitBase's 2D1F2V Shakhov step reads an undefined b, step_1f.jl:194)."""
+    cfg = cavity_2d2f2v(space="2d1f2v", flux="kfvs", inK=0.0)
+    cfg.update(over)
+    return cfg
+
+
 def shock_1d1f3v(**over) -> dict:
     cfg = dict(matter="gas", case="shock", space="1d1f3v", flux="kfvs", collision="bgk", nSpecies=1,
                interpOrder=2, limiter="vanleer", boundary="fix", cfl=0.95, maxTime=1e9, hasForce=False,
@@ -97,8 +105,8 @@ SERVE_SEED = 20260318
 
 
 def serving_requests(n: int = 48, rate: float = 100.0, seed: int = SERVE_SEED, mix=(0.75, 0.15, 0.10),
-                     cavity_cells=(32, 48), cavity_vel=(16, 24), v3_cells: int = 24, cavity_steps: int = 100,
-                     v3_steps: int = 40):
+                     cavity_cells=(32, 48, 64, 96, 128), cavity_vel=(16, 24, 32, 48), v3_cells: int = 24,
+                     cavity_steps: int = 100, v3_steps: int = 40):
     """The serving stream: Poisson arrivals at `rate` requests/s mixing
       * 1D shock tubes of the batched workload (run to t_end = 0.2, KitBase's nt = floor(T/dt) + 1),
       * lid-driven cavities (2D2F2V UGKS) at cavity_cells^2 cells x cavity_vel^2 velocities

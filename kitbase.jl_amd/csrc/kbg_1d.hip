@@ -6,6 +6,7 @@
 //    phases kept separate exactly as KitBase runs them, with slopes and face fluxes stored in HBM.
 //    These back the phase-level entry points and handle any velocity count (1F3V included).
 #include "kbg_device.h"
+#include "kbg_sweep.h"
 #include "kbg_types.h"
 
 namespace kbg {
@@ -233,6 +234,23 @@ template <class T> __global__ __launch_bounds__(1024) void k_wall1d(const Solve1
     const T* WT = (const T*)S.wt;
     const double* bc = S.bcprim + side * 3;
     const double dt = S.dt, K = S.K;
+    // the fused FP32 sweep stores the departure from M~(prim) (kbg_sweep.h): add it back
+    const bool del = fused && S.delta;
+    MaxwCoef mc{};
+    if (del) {
+        double p[3];
+        conserve_prim<3>(S.w[buf] + (size_t)k * 3, S.gam, p);
+        mc = maxw_coef(p, K);
+    }
+    const VChunk<float>* vtf = (const VChunk<float>*)S.vtab;
+    auto hv = [&](int q, int iv) -> T {
+        T x = h[q * nv + iv];
+        if (del) {
+            const float M = maxw_at(vtf[iv / kVChunk], mc.A, mc.nl2, mc.U, iv % kVChunk);
+            x += T(q == 0 ? M : maxw_b(M, mc.kb));
+        }
+        return x;
+    };
     double* fw = fused ? S.bfw + side * 3 : S.fw + (size_t)(side == 0 ? 0 : S.nx) * 3;
     T* ff = fused ? (T*)S.bff + (size_t)side * RS : (T*)S.ff + (size_t)(side == 0 ? 0 : S.nx) * RS;
     const T Uw = T(bc[1]), lw = T(bc[2]);
@@ -241,7 +259,7 @@ template <class T> __global__ __launch_bounds__(1024) void k_wall1d(const Solve1
         const T x = U[iv], wq = WT[iv];
         const bool d = x * T(rot) >= T(0);
         const T c = x - Uw;
-        if (!d) a0.add(wq * x * h[iv]);
+        if (!d) a0.add(wq * x * hv(0, iv));
         else a1.add(wq * x * texp(-lw * (c * c)));
     }
     double r[2] = {a0.val(), a1.val()};
@@ -255,8 +273,8 @@ template <class T> __global__ __launch_bounds__(1024) void k_wall1d(const Solve1
         const bool d = x * T(rot) >= T(0);
         const T c = x - Uw;
         const T H = A * texp(-lw * (c * c));
-        const T hW = d ? H : h[iv];
-        const T bW = d ? H * kb : h[nv + iv];
+        const T hW = d ? H : hv(0, iv);
+        const T bW = d ? H * kb : hv(1, iv);
         m0.add(wq * x * hW);
         m1.add(wq * (x * x) * hW);
         m2.add(wq * x * (hW * (x * x)));
@@ -317,6 +335,7 @@ template <class T, int NF, int VD> __global__ __launch_bounds__(1024) void k_upd
     double wn[NW], pn[NW];
     for (int m = 0; m < NW; m++) wn[m] = wo[m] + (fwL[m] - fwR[m]) / dxc;
     conserve_prim<NW>(wn, S.gam, pn);
+    if (threadIdx.x == 0) flag_health(S.health, prim_health<NW>(pn));
     const double lamn = pn[NW - 1];
     const double tau = vhs_tau(pn[0], lamn, S.mu, S.om);
     const double r = dt / tau;

@@ -341,6 +341,7 @@ template <class T> __global__ __launch_bounds__(256) void k_update2d(const Solve
     double wn[4], pn[4];
     for (int m = 0; m < 4; m++) wn[m] = wo[m] + (fwL[m] - fwR[m] + fwD[m] - fwU[m]) / ds;
     conserve_prim<4>(wn, S.gam, pn);
+    if (threadIdx.x == 0) flag_health(S.health, prim_health<4>(pn));
     const double tau = vhs_tau(pn[0], pn[3], S.mu, S.om);
     const double r = dt / tau;
     const T rT = T(r), inv = T(1.0 / (1.0 + r)), rds = T(1.0 / ds);

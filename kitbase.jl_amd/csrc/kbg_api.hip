@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <initializer_list>
 #include <map>
 #include <string>
 #include <tuple>
@@ -67,14 +68,16 @@ struct kbg_ctx {
     size_t pref2_cap = 0;
     int sweep_steps = 2;  // time steps per fused 1D sweep launch where the solves allow two
     int sweep_vw = 0;     // most velocities per sweep wavefront (0: the kernel's width)
-    // HIP graphs of a batch's steps: [0] a step pair (parities 0, 1), [1] one step (parity 0),
-    // captured for the launch plan whose signature is gsig
+    // HIP graphs of a batch's steps, captured for the launch plan whose signature is gsig:
+    // [p << 1 | 0] a round of two steps, [p << 1 | 1] a round of one step, where p is the buffer
+    // parity of the sweep launches that take two steps per launch at the round's start (launches of
+    // one step per launch start every round at parity 0)
     bool graphs = false;
     std::string gsig;
-    hipGraph_t graph[2] = {};
-    hipGraphExec_t gexec[2] = {};
+    hipGraph_t graph[4] = {};
+    hipGraphExec_t gexec[4] = {};
     void drop_graphs() {
-        for (int k = 0; k < 2; k++) {
+        for (int k = 0; k < 4; k++) {
             if (gexec[k]) (void)hipGraphExecDestroy(gexec[k]);
             if (graph[k]) (void)hipGraphDestroy(graph[k]);
             gexec[k] = nullptr;
@@ -91,6 +94,7 @@ struct kbg_solve {
     size_t esz = 4;
     int nw = 3, N = 0, RS = 0;
     bool fused = false;   // stepped by the fused sweep (cell-fastest f)
+    bool rep_full = false;  // FP32 swept solve whose f buffer holds f, not its departure from M~(prim)
     int thr2 = 0;         // 2D: threads per workgroup of the fused step (0: phase kernels)
     int sweep_nw = 0;     // wavefronts per sweep workgroup
     double vtab_dt = -1;  // dt the velocity table was built for
@@ -206,6 +210,50 @@ int get_f1d(kbg_solve* s, double* dst, const void* src) {
         HIPCHK(hipStreamSynchronize(s->ctx->stream));
         from_layout<float>(S, s->N, tmp.data(), dst);
     }
+    return KBG_OK;
+}
+
+// FP32 swept solves store the departure f - M~(prim): host FP64 f -> departure in buffers `bufs`
+// (prim must be on the device), through an FP64 staging copy in the swept layout
+int put_f1d_delta(kbg_solve* s, const double* src, std::initializer_list<int> bufs) {
+    const Solve1D& S = s->h1;
+    const size_t n = f_elems_1d(S, s->N);
+    std::vector<double> tmp(n, 0.0);
+    to_layout<double>(S, s->N, src, tmp.data());
+    double* stage = nullptr;
+    HIPCHK(hipMalloc((void**)&stage, n * 8));
+    hipError_t e = hipMemcpyAsync(stage, tmp.data(), n * 8, hipMemcpyHostToDevice, s->ctx->stream);
+    for (int b : bufs)
+        if (e == hipSuccess) e = kbg::launch_delta1d(S, s->d1, stage, 0, b, s->ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->ctx->stream);
+    (void)hipFree(stage);
+    if (e != hipSuccess) return fail(KBG_ERR_HIP, std::string("departure upload: ") + hipGetErrorString(e));
+    s->rep_full = false;
+    return KBG_OK;
+}
+
+// f of buffer `buf` as FP64 on the host, exactly M~ + departure
+int get_f1d_delta(kbg_solve* s, double* dst, int buf) {
+    const Solve1D& S = s->h1;
+    const size_t n = f_elems_1d(S, s->N);
+    std::vector<double> tmp(n);
+    double* stage = nullptr;
+    HIPCHK(hipMalloc((void**)&stage, n * 8));
+    hipError_t e = hipMemsetAsync(stage, 0, n * 8, s->ctx->stream);
+    if (e == hipSuccess) e = kbg::launch_delta1d(S, s->d1, stage, 1, buf, s->ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(tmp.data(), stage, n * 8, hipMemcpyDeviceToHost, s->ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->ctx->stream);
+    (void)hipFree(stage);
+    if (e != hipSuccess) return fail(KBG_ERR_HIP, std::string("departure download: ") + hipGetErrorString(e));
+    from_layout<double>(S, s->N, tmp.data(), dst);
+    return KBG_OK;
+}
+
+// the phase-level kernels work on f itself: convert a departure-form solve once (in place, FP32)
+int to_full_rep(kbg_solve* s) {
+    if (s->d.dim != 1 || !s->h1.delta || s->rep_full) return KBG_OK;
+    HIPCHK(kbg::launch_delta1d(s->h1, s->d1, nullptr, 2, s->h1.cur, s->ctx->stream));
+    s->rep_full = true;
     return KBG_OK;
 }
 
@@ -345,6 +393,7 @@ void fill1d(kbg_solve* s) {
     // the fused sweep takes one velocity dimension and up to kSweepMaxWaves wavefronts of velocities
     const int N = d.nx + 2 * d.ng;
     S.swept = swept_1d(d);
+    S.delta = S.swept && d.precision == KBG_FP32;
     if (S.swept) {
         S.np = (N + 31) / 32 * 32;
         S.fcs = 1;
@@ -466,6 +515,7 @@ void build_items(kbg_solve* s, int seg, int solve_index, int nw, std::vector<Wor
 
 int step_phase_1d(kbg_solve* s, double dt, double* residual, hipStream_t st);
 int step_phase_2d(kbg_solve* s, double dt, double* residual, hipStream_t st);
+int batch_step_impl(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double* dts, int nsteps, double* residuals);
 
 }  // namespace
 
@@ -678,6 +728,9 @@ int kbg_solve_create(kbg_ctx* ctx, const kbg_desc* desc, kbg_solve** out) {
         s->rcell = (double*)p;
         if ((rc = dev_alloc(s, &p, 8 * 8))) return bail(rc);
         s->res = (double*)p;
+        if ((rc = dev_alloc(s, &p, 8))) return bail(rc);
+        HIPCHK(hipMemset(p, 0, 8));
+        S.health = (int*)p;
         if ((rc = dev_alloc(s, &p, sizeof(Solve1D)))) return bail(rc);
         s->d1 = (Solve1D*)p;
         s->fused = S.swept != 0;
@@ -687,6 +740,7 @@ int kbg_solve_create(kbg_ctx* ctx, const kbg_desc* desc, kbg_solve** out) {
             if ((rc = dev_alloc(s, &s->vtab, vb))) return bail(rc);
             HIPCHK(hipMemset(s->vtab, 0, vb));
             S.vtab = s->vtab;
+            if ((rc = ensure_vtab(s, 0.0))) return bail(rc);
         }
         if ((rc = push1d(s))) return bail(rc);
     } else {
@@ -751,6 +805,9 @@ int kbg_solve_create(kbg_ctx* ctx, const kbg_desc* desc, kbg_solve** out) {
         s->thr2 = FX * 2 * d.nv * s->esz < (1ull << 31) && FY * 2 * d.nv * s->esz < (1ull << 31) ? kbg::step2d_threads(s->fp64, d.nv) : 0;
         if ((rc = dev_alloc(s, &p, 8 * 8))) return bail(rc);
         s->res = (double*)p;
+        if ((rc = dev_alloc(s, &p, 8))) return bail(rc);
+        HIPCHK(hipMemset(p, 0, 8));
+        S.health = (int*)p;
         if ((rc = dev_alloc(s, &p, sizeof(Solve2D)))) return bail(rc);
         s->d2 = (Solve2D*)p;
         // slope buffers of the phase-level calls are allocated on first use, or now when the fused
@@ -773,17 +830,30 @@ int kbg_solve_destroy(kbg_solve* s) {
     return KBG_OK;
 }
 
+int kbg_solve_health(kbg_solve* s, int* flags, int clear) {
+    if (!s || !flags) return fail(KBG_ERR_ARG, "NULL argument");
+    int* h = s->d.dim == 1 ? s->h1.health : s->h2.health;
+    HIPCHK(hipMemcpyAsync(flags, h, sizeof(int), hipMemcpyDeviceToHost, s->ctx->stream));
+    if (clear) HIPCHK(hipMemsetAsync(h, 0, sizeof(int), s->ctx->stream));
+    HIPCHK(hipStreamSynchronize(s->ctx->stream));
+    return KBG_OK;
+}
+
 int kbg_upload(kbg_solve* s, const double* w, const double* prim, const double* f) {
     if (!s || !w || !prim || !f) return fail(KBG_ERR_ARG, "NULL argument");
     int rc;
     const size_t N = s->N;
     if (s->d.dim == 1) {
         Solve1D& S = s->h1;
-        for (int b = 0; b < 2; b++) {
-            if ((rc = put_f64(s, S.w[b], w, N * s->nw))) return rc;
-            if ((rc = put_f1d(s, S.f[b], f))) return rc;
-        }
         if ((rc = put_f64(s, S.prim, prim, N * s->nw))) return rc;
+        for (int b = 0; b < 2; b++)
+            if ((rc = put_f64(s, S.w[b], w, N * s->nw))) return rc;
+        if (S.delta) {
+            if ((rc = push1d(s))) return rc;
+            return put_f1d_delta(s, f, {0, 1});
+        }
+        for (int b = 0; b < 2; b++)
+            if ((rc = put_f1d(s, S.f[b], f))) return rc;
         return KBG_OK;
     }
     Solve2D& S = s->h2;
@@ -800,7 +870,13 @@ int kbg_download(kbg_solve* s, double* w, double* prim, double* f) {
         Solve1D& S = s->h1;
         if (w && (rc = get_f64(s, w, S.w[S.cur], N * s->nw))) return rc;
         if (prim && (rc = get_f64(s, prim, S.prim, N * s->nw))) return rc;
-        if (f && (rc = get_f1d(s, f, S.f[S.cur]))) return rc;
+        if (f) {
+            if (S.delta && !s->rep_full) {
+                if ((rc = push1d(s)) || (rc = get_f1d_delta(s, f, S.cur))) return rc;
+            } else if ((rc = get_f1d(s, f, S.f[S.cur]))) {
+                return rc;
+            }
+        }
         return KBG_OK;
     }
     Solve2D& S = s->h2;
@@ -892,7 +968,7 @@ int kbg_reconstruct(kbg_solve* s) {
     if (!s) return fail(KBG_ERR_ARG, "NULL solve");
     int rc;
     if (s->d.dim == 1) {
-        if ((rc = ensure_phase_buffers(s)) || (rc = push1d(s))) return rc;
+        if ((rc = ensure_phase_buffers(s)) || (rc = push1d(s)) || (rc = to_full_rep(s))) return rc;
         HIPCHK(kbg::launch_recon1d(s->fp64, s->h1, s->d1, s->ctx->stream));
     } else {
         if ((rc = ensure_phase_buffers(s)) || (rc = push2d(s))) return rc;
@@ -908,7 +984,7 @@ int kbg_evolve(kbg_solve* s, double dt) {
     if (s->d.dim == 1) {
         if ((rc = ensure_phase_buffers(s))) return rc;
         s->h1.dt = dt;
-        if ((rc = push1d(s))) return rc;
+        if ((rc = push1d(s)) || (rc = to_full_rep(s))) return rc;
         HIPCHK(kbg::launch_evolve1d(s->fp64, s->h1, s->d1, s->ctx->stream));
         HIPCHK(kbg::launch_wall1d(s->fp64, s->h1, s->d1, 0, 0, s->ctx->stream));
         if ((rc = sync_boundary_faces(s))) return rc;
@@ -928,7 +1004,7 @@ int kbg_update(kbg_solve* s, double dt, double* residual) {
     if (s->d.dim == 1) {
         if ((rc = ensure_phase_buffers(s))) return rc;
         s->h1.dt = dt;
-        if ((rc = push1d(s))) return rc;
+        if ((rc = push1d(s)) || (rc = to_full_rep(s))) return rc;
         rc = step_phase_1d(s, dt, residual, s->ctx->stream);
     } else {
         if ((rc = ensure_phase_buffers(s))) return rc;
@@ -970,6 +1046,17 @@ int kbg_step(kbg_solve* s, double dt, int nsteps, double* residual) {
 }
 
 int kbg_batch_step(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double* dts, int nsteps, double* residuals) {
+    return batch_step_impl(ctx, solves, n, dts, nsteps, residuals);
+}
+
+}  // extern "C"
+
+namespace {
+
+// kbg_batch_step.  The batch's device staging (1D descriptors, work items, packs, residual partial
+// rows; 2D descriptors and prefix sums) lives in the context's buffers, grown with hipMalloc when a
+// batch outgrows them.
+int batch_step_impl(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double* dts, int nsteps, double* residuals) {
     if (!ctx || (n > 0 && (!solves || !dts))) return fail(KBG_ERR_ARG, "NULL argument");
     if (nsteps <= 0 || n <= 0) return KBG_OK;
     hipStream_t st = ctx->stream;
@@ -1029,33 +1116,21 @@ int kbg_batch_step(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double* 
             pref2.insert(pref2.end(), ip.begin(), ip.end());
             launches2.push_back(L);
         }
-        if (!descs2.empty()) {
-            if (descs2.size() > ctx->descs2_cap) {
-                (void)hipFree(ctx->d_descs2);
-                ctx->d_descs2 = nullptr;
-                ctx->descs2_cap = descs2.size() * 2;
-                HIPCHK(hipMalloc((void**)&ctx->d_descs2, ctx->descs2_cap * sizeof(Solve2D)));
-            }
-            if (pref2.size() > ctx->pref2_cap) {
-                (void)hipFree(ctx->d_pref2);
-                ctx->d_pref2 = nullptr;
-                ctx->pref2_cap = pref2.size() * 2;
-                HIPCHK(hipMalloc((void**)&ctx->d_pref2, ctx->pref2_cap * sizeof(int)));
-            }
-            HIPCHK(hipMemcpyAsync(ctx->d_descs2, descs2.data(), descs2.size() * sizeof(Solve2D), hipMemcpyHostToDevice, st));
-            HIPCHK(hipMemcpyAsync(ctx->d_pref2, pref2.data(), pref2.size() * sizeof(int), hipMemcpyHostToDevice, st));
-            HIPCHK(hipStreamSynchronize(st));
-        }
     }
-    // two time steps per sweep launch when every fused 1D solve allows it (KFVS, no boundary-face
-    // buffers: no walls, no stepped cell next to a ghost-less boundary); not under graph replay
-    bool two = ctx->sweep_steps == 2 && nsteps >= 2 && !fused.empty() && !(ctx->graphs && !residuals);
+    // two time steps per sweep launch for each fused 1D solve that allows it (KFVS, no boundary-face
+    // buffers: no walls, no stepped cell next to a ghost-less boundary); not under graph replay.  The
+    // choice is the solve's own, so a solve advances in the same step pairs (counted from the first
+    // step of the call) in any batch: a two-step pass may round differently from two one-step passes.
+    const bool two_ok = ctx->sweep_steps == 2;
+    std::vector<char> two_of(n, 0);
+    bool two = false;  // some launch takes two steps per round
     for (int i : fused) {
         const Solve1D& S = solves[i]->h1;
-        if (S.flux != kbg::KFVS || S.wallL || S.wallR || (S.faceL_buf && S.upd_lo == 1) || (S.faceR_buf && S.upd_hi == S.nx))
-            two = false;
+        two_of[i] = two_ok && S.flux == kbg::KFVS && !S.wallL && !S.wallR && !(S.faceL_buf && S.upd_lo == 1) &&
+                    !(S.faceR_buf && S.upd_hi == S.nx);
+        two = two || two_of[i];
     }
-    const int seg = std::min(std::max(1, ctx->seg_cells), two ? kbg::kSweepOut2 : kbg::kSweepOut);
+    auto seg_of = [&](int i) { return std::min(std::max(1, ctx->seg_cells), two_of[i] ? kbg::kSweepOut2 : kbg::kSweepOut); };
     // Independent units -- fused 1D solves, fused 2D groups and solves stepped by the phase kernels
     // -- are spread over the streams by work (cells x velocities x distributions), heaviest first;
     // each stream advances its own share.
@@ -1095,17 +1170,22 @@ int kbg_batch_step(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double* 
         int nf, flux, wg_waves, stream;
         size_t pack0, npacks;
         std::vector<int> members;  // solve indices (for the wall kernels)
+        bool two;                  // two time steps per launch in a round of two
+        int flips;                 // sweep launches enqueued so far (the f / w buffer parity)
     };
     std::vector<Solve1D> descs;
     std::vector<int> desc_of(n, -1);
     std::vector<WorkItem> items;
     std::vector<kbg::Pack> packs;
     std::vector<Launch> launches;
-    std::map<std::tuple<int, int, int, int>, std::vector<int>> keyed;  // (stream, fp64, nf, flux) -> solves
-    for (int i : fused) keyed[{stream_of[i], solves[i]->fp64 ? 1 : 0, solves[i]->d.nf, solves[i]->d.flux}].push_back(i);
+    std::map<std::tuple<int, int, int, int, int>, std::vector<int>> keyed;  // (stream, fp64, nf, flux, two) -> solves
+    for (int i : fused)
+        keyed[{stream_of[i], solves[i]->fp64 ? 1 : 0, solves[i]->d.nf, solves[i]->d.flux, two_of[i] ? 1 : 0}].push_back(i);
+    std::vector<int> launch_of(n, -1);
     for (auto& kv : keyed) {
         Launch L{std::get<1>(kv.first) == 1, std::get<2>(kv.first), std::get<3>(kv.first), 1, std::get<0>(kv.first),
-                 packs.size(), 0, kv.second};
+                 packs.size(), 0, kv.second, std::get<4>(kv.first) == 1, 0};
+        for (int i : kv.second) launch_of[i] = (int)launches.size();
         std::map<int, std::vector<int>> by_nw;  // nw -> item indices, in solve order
         for (int i : kv.second) {
             kbg_solve* s = solves[i];
@@ -1119,7 +1199,7 @@ int kbg_batch_step(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double* 
             int nw = s->sweep_nw;
             if (ctx->sweep_vw > 0)
                 while (nw < kbg::kSweepMaxWaves && (s->d.nv + nw - 1) / nw > ctx->sweep_vw) nw *= 2;
-            build_items(s, seg, desc_of[i], nw, items);
+            build_items(s, seg_of(i), desc_of[i], nw, items);
             for (size_t it = i0; it < items.size(); it++) by_nw[nw].push_back((int)it);
             L.wg_waves = std::max(L.wg_waves, nw);
         }
@@ -1142,28 +1222,69 @@ int kbg_batch_step(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double* 
         L.npacks = packs.size() - L.pack0;
         launches.push_back(std::move(L));
     }
+    // device staging of the batch: the context's buffers (grown on demand)
+    Solve1D* d_descs = nullptr;
+    WorkItem* d_items = nullptr;
+    double* d_rpart = nullptr;
+    kbg::Pack* d_packs = nullptr;
+    Solve2D* d_descs2 = nullptr;
+    int* d_pref2 = nullptr;
+    if (descs.size() > ctx->descs_cap) {
+        (void)hipFree(ctx->d_descs);
+        ctx->d_descs = nullptr;
+        ctx->descs_cap = descs.size() * 2;
+        HIPCHK(hipMalloc((void**)&ctx->d_descs, ctx->descs_cap * sizeof(Solve1D)));
+    }
+    if (items.size() > ctx->items_cap) {
+        (void)hipFree(ctx->d_items);
+        (void)hipFree(ctx->d_rpart);
+        ctx->d_items = nullptr;
+        ctx->d_rpart = nullptr;
+        ctx->items_cap = items.size() * 2;
+        HIPCHK(hipMalloc((void**)&ctx->d_items, ctx->items_cap * sizeof(WorkItem)));
+        HIPCHK(hipMalloc((void**)&ctx->d_rpart, ctx->items_cap * 12 * sizeof(double)));
+    }
+    if (packs.size() > ctx->packs_cap) {
+        (void)hipFree(ctx->d_packs);
+        ctx->d_packs = nullptr;
+        ctx->packs_cap = packs.size() * 2;
+        HIPCHK(hipMalloc((void**)&ctx->d_packs, ctx->packs_cap * sizeof(kbg::Pack)));
+    }
+    if (descs2.size() > ctx->descs2_cap) {
+        (void)hipFree(ctx->d_descs2);
+        ctx->d_descs2 = nullptr;
+        ctx->descs2_cap = descs2.size() * 2;
+        HIPCHK(hipMalloc((void**)&ctx->d_descs2, ctx->descs2_cap * sizeof(Solve2D)));
+    }
+    if (pref2.size() > ctx->pref2_cap) {
+        (void)hipFree(ctx->d_pref2);
+        ctx->d_pref2 = nullptr;
+        ctx->pref2_cap = pref2.size() * 2;
+        HIPCHK(hipMalloc((void**)&ctx->d_pref2, ctx->pref2_cap * sizeof(int)));
+    }
+    d_descs = ctx->d_descs;
+    d_items = ctx->d_items;
+    d_rpart = ctx->d_rpart;
+    d_packs = ctx->d_packs;
+    d_descs2 = ctx->d_descs2;
+    d_pref2 = ctx->d_pref2;
     if (!descs.empty()) {
-        if (descs.size() > ctx->descs_cap) {
-            (void)hipFree(ctx->d_descs);
-            ctx->descs_cap = descs.size() * 2;
-            HIPCHK(hipMalloc((void**)&ctx->d_descs, ctx->descs_cap * sizeof(Solve1D)));
+        HIPCHK(hipMemcpyAsync(d_descs, descs.data(), descs.size() * sizeof(Solve1D), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(d_items, items.data(), items.size() * sizeof(WorkItem), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(d_packs, packs.data(), packs.size() * sizeof(kbg::Pack), hipMemcpyHostToDevice, st));
+    }
+    if (!descs2.empty()) {
+        HIPCHK(hipMemcpyAsync(d_descs2, descs2.data(), descs2.size() * sizeof(Solve2D), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(d_pref2, pref2.data(), pref2.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    }
+    if (!descs.empty() || !descs2.empty()) HIPCHK(hipStreamSynchronize(st));
+    // fused FP32 solves left holding f by phase-level calls go back to departures (in place)
+    for (int i : fused) {
+        kbg_solve* s = solves[i];
+        if (s->h1.delta && s->rep_full) {
+            HIPCHK(kbg::launch_delta1d(s->h1, d_descs + desc_of[i], nullptr, 3, s->h1.cur, st));
+            s->rep_full = false;
         }
-        if (items.size() > ctx->items_cap) {
-            (void)hipFree(ctx->d_items);
-            (void)hipFree(ctx->d_rpart);
-            ctx->items_cap = items.size() * 2;
-            HIPCHK(hipMalloc((void**)&ctx->d_items, ctx->items_cap * sizeof(WorkItem)));
-            HIPCHK(hipMalloc((void**)&ctx->d_rpart, ctx->items_cap * 12 * sizeof(double)));
-        }
-        if (packs.size() > ctx->packs_cap) {
-            (void)hipFree(ctx->d_packs);
-            ctx->packs_cap = packs.size() * 2;
-            HIPCHK(hipMalloc((void**)&ctx->d_packs, ctx->packs_cap * sizeof(kbg::Pack)));
-        }
-        HIPCHK(hipMemcpyAsync(ctx->d_descs, descs.data(), descs.size() * sizeof(Solve1D), hipMemcpyHostToDevice, st));
-        HIPCHK(hipMemcpyAsync(ctx->d_items, items.data(), items.size() * sizeof(WorkItem), hipMemcpyHostToDevice, st));
-        HIPCHK(hipMemcpyAsync(ctx->d_packs, packs.data(), packs.size() * sizeof(kbg::Pack), hipMemcpyHostToDevice, st));
-        HIPCHK(hipStreamSynchronize(st));
     }
     // stream k > 0 is side stream k - 1; they fork from and join back into the context stream
     const int nside = nstreams - 1;
@@ -1183,29 +1304,36 @@ int kbg_batch_step(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double* 
         }
         return KBG_OK;
     };
-    // one round of every unit on its stream: nst time steps (1, or 2 for the fused 1D sweep when
-    // `two` holds; the other units are launched nst times); par is the round parity (which f / w
-    // buffer the fused 1D sweep reads), last marks the round whose residual rows are kept
-    auto enqueue_round = [&](int par, int nst, bool last_round) -> int {
+    // one round of every unit on its stream: nst time steps (1 or 2; a fused 1D sweep launch whose
+    // solves allow it takes both in one pass, the other units are launched nst times); each sweep
+    // launch reads the f / w buffers of its own parity (launches so far), last marks the round whose
+    // residual rows are kept
+    auto enqueue_round = [&](int nst, bool last_round) -> int {
         int e;
-        for (const Launch& L : launches) {
-            for (int di : L.members) {
-                const Solve1D& S = descs[desc_of[di]];
-                if (S.wallL || S.wallR)
-                    HIPCHK(kbg::launch_wall1d(solves[di]->fp64, S, ctx->d_descs + desc_of[di], 1, par, sst[L.stream]));
+        for (Launch& L : launches) {
+            const int per = L.two ? nst : 1;  // steps per launch
+            for (int ks = 0; ks < nst; ks += per) {
+                const int par = L.flips & 1;
+                for (int di : L.members) {
+                    const Solve1D& S = descs[desc_of[di]];
+                    if (S.wallL || S.wallR)
+                        HIPCHK(kbg::launch_wall1d(solves[di]->fp64, S, d_descs + desc_of[di], 1, par, sst[L.stream]));
+                }
+                const bool keep = last_round && residuals && ks + per == nst;
+                HIPCHK(kbg::launch_sweep1d(L.fp64, L.nf, L.flux, L.wg_waves, d_descs, d_items, d_packs + L.pack0, (int)L.npacks,
+                                           par, keep ? d_rpart : nullptr, per, sst[L.stream]));
+                L.flips++;
             }
-            HIPCHK(kbg::launch_sweep1d(L.fp64, L.nf, L.flux, L.wg_waves, ctx->d_descs, ctx->d_items, ctx->d_packs + L.pack0,
-                                       (int)L.npacks, par, (last_round && residuals) ? ctx->d_rpart : nullptr, nst, sst[L.stream]));
         }
         for (int ks = 0; ks < nst; ks++) {
         const bool last = last_round && ks == nst - 1;
         for (size_t g = 0; g < launches2.size(); g++) {
             const Launch2& L = launches2[g];
-            const Solve2D* dd = ctx->d_descs2 + L.desc0;
+            const Solve2D* dd = d_descs2 + L.desc0;
             hipStream_t ss = sst[stream2[g]];
-            HIPCHK(kbg::launch_face2d(true, L.flux, dd, ctx->d_pref2 + L.ipref0, L.n, L.ninterior, ss));
-            HIPCHK(kbg::launch_flux2d(L.fp64, L.flux, L.threads, dd, ctx->d_pref2 + L.fpref0, L.n, L.nfaces, ss));
-            HIPCHK(kbg::launch_cell2d(L.fp64, L.threads, dd, ctx->d_pref2 + L.cpref0, L.n, L.ncells, (last && residuals) ? 1 : 0, ss));
+            HIPCHK(kbg::launch_face2d(true, L.flux, dd, d_pref2 + L.ipref0, L.n, L.ninterior, ss));
+            HIPCHK(kbg::launch_flux2d(L.fp64, L.flux, L.threads, dd, d_pref2 + L.fpref0, L.n, L.nfaces, ss));
+            HIPCHK(kbg::launch_cell2d(L.fp64, L.threads, dd, d_pref2 + L.cpref0, L.n, L.ncells, (last && residuals) ? 1 : 0, ss));
         }
         for (int i : others) {
             kbg_solve* s = solves[i];
@@ -1225,8 +1353,8 @@ int kbg_batch_step(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double* 
         }
         return KBG_OK;
     };
-    // Without residuals the steps are replayed from HIP graphs: a step pair and a single step,
-    // captured once per launch plan (the signature below: every launch's shape, stream and
+    // Without residuals the steps are replayed from HIP graphs: rounds of two steps and of one step
+    // at both parities of the two-step launches, captured once per launch plan (the signature below: every launch's shape, stream and
     // arguments; descriptors are read from device memory, re-uploaded by every call) and reused
     // by later calls on the same batch.
     const bool use_graph = ctx->graphs && !residuals && nsteps >= 2;
@@ -1235,7 +1363,7 @@ int kbg_batch_step(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double* 
         auto put = [&](const void* p, size_t b) { sig.append(static_cast<const char*>(p), b); };
         auto puti = [&](long long v) { put(&v, sizeof v); };
         puti(nstreams);
-        const void* bufs[] = {ctx->d_descs, ctx->d_items, ctx->d_packs, ctx->d_descs2, ctx->d_pref2};
+        const void* bufs[] = {d_descs, d_items, d_packs, d_descs2, d_pref2};
         put(bufs, sizeof bufs);
         for (Solve1D d : descs) {
             d.cur = 0;  // the parity base flips between calls; the kernels read it from the descriptor
@@ -1245,7 +1373,7 @@ int kbg_batch_step(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double* 
         put(packs.data(), packs.size() * sizeof(kbg::Pack));
         for (const Launch& L : launches) {
             for (long long v : {(long long)L.fp64, (long long)L.nf, (long long)L.flux, (long long)L.wg_waves, (long long)L.stream,
-                                (long long)L.pack0, (long long)L.npacks})
+                                (long long)L.pack0, (long long)L.npacks, (long long)L.two})
                 puti(v);
             for (int di : L.members) {
                 puti(di);
@@ -1276,13 +1404,14 @@ int kbg_batch_step(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double* 
                 put(&s->h2, sizeof s->h2);
             }
         }
-        if (sig != ctx->gsig || !ctx->gexec[0] || !ctx->gexec[1]) {
+        if (sig != ctx->gsig || !ctx->gexec[0] || !ctx->gexec[1] || !ctx->gexec[2] || !ctx->gexec[3]) {
             ctx->drop_graphs();
-            for (int which = 0; which < 2; which++) {
+            for (int which = 0; which < 4; which++) {
                 HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+                for (Launch& L : launches) L.flips = L.two ? which >> 1 : 0;
                 int e = fork();
-                if (!e) e = enqueue_round(0, 1, false);
-                if (!e && which == 0) e = enqueue_round(1, 1, false);
+                if (!e) e = enqueue_round((which & 1) || !two ? 1 : 2, false);
+                if (!e && !(which & 1) && !two) e = enqueue_round(1, false);
                 if (!e) e = join();
                 hipGraph_t g = nullptr;
                 const hipError_t ce = hipStreamEndCapture(st, &g);
@@ -1298,17 +1427,18 @@ int kbg_batch_step(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double* 
         }
     }
     HIPCHK(hipEventRecord(ctx->ev[KBG_EV_STEPS_BEGIN], st));
-    int rounds = use_graph ? nsteps : 0;  // sweep launches (buffer flips) of the fused 1D solves
+    for (Launch& L : launches) L.flips = 0;
     if (use_graph) {
-        for (int t = 0; t + 1 < nsteps; t += 2) HIPCHK(hipGraphLaunch(ctx->gexec[0], st));
-        if (nsteps & 1) HIPCHK(hipGraphLaunch(ctx->gexec[1], st));
+        const int pairs = nsteps / 2;
+        for (int t = 0; t < pairs; t++) HIPCHK(hipGraphLaunch(ctx->gexec[(t & 1) << 1], st));
+        if (nsteps & 1) HIPCHK(hipGraphLaunch(ctx->gexec[(pairs & 1) << 1 | 1], st));
+        for (Launch& L : launches) L.flips = L.two ? pairs + (nsteps & 1) : nsteps;
     } else {
         if ((rc = fork())) return rc;
         for (int t = 0; t < nsteps;) {
             const int nst = two && nsteps - t >= 2 ? 2 : 1;
-            if ((rc = enqueue_round(rounds & 1, nst, t + nst == nsteps))) return rc;
+            if ((rc = enqueue_round(nst, t + nst == nsteps))) return rc;
             t += nst;
-            rounds++;
         }
         if ((rc = join())) return rc;
     }
@@ -1320,7 +1450,7 @@ int kbg_batch_step(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double* 
             if (desc_of[i] >= 0) fused_idx.push_back(i);
         if (!fused_idx.empty()) {
             std::vector<double> rp(items.size() * 12);
-            HIPCHK(hipMemcpyAsync(rp.data(), ctx->d_rpart, rp.size() * 8, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipMemcpyAsync(rp.data(), d_rpart, rp.size() * 8, hipMemcpyDeviceToHost, st));
             HIPCHK(hipStreamSynchronize(st));
             std::vector<double> acc(descs.size() * 12, 0.0);
             for (size_t it = 0; it < items.size(); it++)
@@ -1339,7 +1469,7 @@ int kbg_batch_step(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double* 
     for (int i = 0; i < n; i++) {
         if (desc_of[i] < 0) continue;
         kbg_solve* s = solves[i];
-        s->h1.cur = (s->h1.cur + rounds) & 1;
+        s->h1.cur = (s->h1.cur + launches[launch_of[i]].flips) & 1;
     }
     // fused 2D solves: residuals from the last step's rows
     if (residuals) {
@@ -1352,10 +1482,6 @@ int kbg_batch_step(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double* 
     HIPCHK(hipStreamSynchronize(st));
     return KBG_OK;
 }
-
-}  // extern "C"
-
-namespace {
 
 // update! on the phase-exact path (kernels k_update1d / k_resid), optional residual download
 int step_phase_1d(kbg_solve* s, double dt, double* residual, hipStream_t st) {

@@ -68,6 +68,18 @@ template <class T> KBG_D T limf(int kind, T sL, T sR) {
     return limiter<T>(kind, sL, sR);
 }
 
+// Failure detection: the health of a cell's new primitive variables -- 1 not finite, 2 rho <= 0,
+// 4 lambda <= 0 (a NaN fails all three).  A bad cell raises these bits in its solve's flag word; the
+// step itself completes.
+template <int N> KBG_HD int prim_health(const double* p) {
+    bool fin = true;
+    for (int m = 0; m < N; m++) fin = fin && isfinite(p[m]);
+    return (fin ? 0 : 1) | (p[0] > 0.0 ? 0 : 2) | (p[N - 1] > 0.0 ? 0 : 4);
+}
+KBG_D void flag_health(int* h, int bits) {
+    if (bits && h) __hip_atomic_fetch_or(h, bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // conserve_prim (theory_macro.jl:185-206) for n = 3 (1D), 4 (2D), 5 (3V)
 template <int N> KBG_HD void conserve_prim(const double* W, double gam, double* p) {
     if constexpr (N == 3) {

@@ -594,6 +594,7 @@ __global__ __launch_bounds__(step2d_max_threads<T>()) void k_cell2d(const Solve2
         const double tau = vhs_tau(pn[0], pn[3], S.mu, S.om);
         const double r = dt / tau;
         if (lane == 0) {
+            flag_health(S.health, prim_health<4>(pn));
             cst[C_RT] = r;
             cst[C_INV] = 1.0 / (1.0 + r);
             cst[C_RDS] = 1.0 / ds;

@@ -96,6 +96,72 @@ template <class T> struct alignas(64) VChunk {
 };
 constexpr int vchunks(int nv) { return nv / kVChunk + 2; }
 
+// The FP32 sweep stores each distribution as its departure from M~(prim), the Maxwellian of the
+// cell's primitive variables (conserve_prim of the w buffer paired with the f buffer) evaluated in
+// FP32: the departure is small where f is near
+// equilibrium, so the FP32 rounding of the stored state (and of the per-step update, done on the
+// departure) shrinks with it.  Every kernel that reads or writes that storage evaluates M~ from the
+// same FP32 coefficients with the same operations in the same order, so a stored f - M~ reads back
+// against bit-identical M~.
+struct MaxwCoef {
+    float A, nl2, U, kb;  // rho (lambda / pi)^(1/2), -lambda log2(e), U, K / (2 lambda)
+};
+KBG_D MaxwCoef maxw_coef(const double* p, double K) {
+    return MaxwCoef{float(p[0] * sqrt(p[2] / kPi)), float(-p[2] * 1.4426950408889634074), float(p[1]), float(K / (2.0 * p[2]))};
+}
+// M~ at the 8 nodes of a velocity chunk: one exp2 per node, or on equally spaced nodes the product
+// recurrence M(u + du) = M(u) R, R <- R Q with R = exp(-lambda du (2 (u - U) + du)), Q = exp(-2 lambda du^2).
+// Compiled without floating-point contraction: a product left unrounded inside a fused multiply-add
+// at one use and rounded at another would give the same node two values.
+template <class CHK> KBG_D void maxw8(CHK& ch, float A, float nl2, float U0, float (&M)[kVChunk]) {
+#pragma clang fp contract(off)
+    // plain loops: the pragma reaches the statements of this body, not those of a lambda in it
+    const float d = ch.du;
+    if (d != 0.f) {
+        const float c0 = ch.u[0] - U0;
+        float m = A * __builtin_amdgcn_exp2f(nl2 * (c0 * c0));
+        float R = __builtin_amdgcn_exp2f((nl2 * d) * __builtin_fmaf(2.f, c0, d));
+        const float Q = __builtin_amdgcn_exp2f((2.f * nl2) * (d * d));
+#pragma unroll
+        for (int j = 0; j < kVChunk; j++) {
+            M[j] = m;
+            if (j + 1 < kVChunk) {
+                m *= R;
+                R *= Q;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < kVChunk; j++) {
+            const float c = ch.u[j] - U0;
+            M[j] = A * __builtin_amdgcn_exp2f(nl2 * (c * c));
+        }
+    }
+}
+// the b distribution's M~ K / (2 lambda): a rounded product (no contraction into a later addition)
+KBG_D float maxw_b(float M, float kb) {
+#pragma clang fp contract(off)
+    return M * kb;
+}
+// the same value at node j of the chunk alone
+template <class CHK> KBG_D float maxw_at(CHK& ch, float A, float nl2, float U0, int j) {
+#pragma clang fp contract(off)
+    const float d = ch.du;
+    if (d != 0.f) {
+        const float c0 = ch.u[0] - U0;
+        float m = A * __builtin_amdgcn_exp2f(nl2 * (c0 * c0));
+        float R = __builtin_amdgcn_exp2f((nl2 * d) * __builtin_fmaf(2.f, c0, d));
+        const float Q = __builtin_amdgcn_exp2f((2.f * nl2) * (d * d));
+        for (int jj = 0; jj < j; jj++) {
+            m *= R;
+            R *= Q;
+        }
+        return m;
+    }
+    const float c = ch.u[j] - U0;
+    return A * __builtin_amdgcn_exp2f(nl2 * (c * c));
+}
+
 // velocities per wavefront and moment-sum counts of a kernel instance
 template <class T, int NF, int FLUX> struct SweepCfg {
     static constexpr int VW = sweep_vw(sizeof(T) == 8, NF);
@@ -186,7 +252,29 @@ KBG_D void sweep1d(const Solve1D& S, const WorkItem& wi, int par, int lane, int 
     // register footprint small; pass 1 needs only the old velocity for the Shakhov heat flux
     const gptr<const double> win = to_global((const double*)S.w[in]);
     const bool shak = SHAK && i > 1 && i < nx;  // boundary cells relax with BGK (update_boundary.jl)
-    const double Uo64 = SHAK ? S.prim[(size_t)kc * 3 + 1] : 0.0;
+    // The cell's primitive variables at the start of the pass, prim^n = conserve_prim(w^n), from the
+    // w buffer, double-buffered like f -- not from the stored prim, which a neighbouring workgroup may
+    // already have overwritten with its halo cells' prim^{n+1} (updated cells store prim =
+    // conserve_prim(w), the initial state KitBase's init_fvm sets the same way)
+    constexpr bool DEL = sizeof(T) == 4;
+    double pq[3] = {1.0, 0.0, 1.0};
+    if (SHAK || DEL) {
+        double wq[3];
+#pragma unroll
+        for (int m = 0; m < 3; m++) wq[m] = win[(size_t)kc * 3 + m];
+        conserve_prim<3>(wq, S.gam, pq);
+    }
+    const double Uo64 = SHAK ? pq[1] : 0.0;
+    // FP32: the stored f of the lane's cell departs from M~(prim^n) (for the second step of a pass,
+    // prim^{n+1} where the first step updated the cell); lanes outside the solve read 0
+    float dA = 0.f, dnl = 0.f, dU = 0.f, dkb = 0.f;
+    if constexpr (DEL) {
+        const MaxwCoef mc = maxw_coef(pq, K);
+        dA = (k >= 0 && k < N) ? mc.A : 0.f;
+        dnl = mc.nl2;
+        dU = mc.U;
+        dkb = mc.kb;
+    }
 
     // ---- face coefficients of the lane's left face (KCU / UGKS), FP64 per lane
     SFace<T> F{};
@@ -280,16 +368,20 @@ KBG_D void sweep1d(const Solve1D& S, const WorkItem& wi, int par, int lane, int 
         T part[NR];  // chunk sums in the storage precision, added to the FP64 totals below
 #pragma unroll
         for (int m = 0; m < NR; m++) part[m] = T(0);
+        float Md[CH];  // M~ the stored departures are taken from
+        if constexpr (DEL) maxw8(ch, dA, dnl, dU, Md);
         static_for<CH>([&](auto jc) {
             constexpr int j = jc;
             const T u = ch.u[j], uu = ch.uu[j];
             const bool up = (ch.upmask >> j) & 1;
-            T fq[NF], sq[NF], fu[NF], su[NF], ffq[NF];
+            T fq[NF], dq[NF], sq[NF], fu[NF], su[NF], ffq[NF];
 #pragma unroll
             for (int q = 0; q < NF; q++) {
                 T f;
                 if constexpr (SS == 0) f = fr[q][j];
-                else f = G[q][c * CH + j];  // the first step's f^{n+1}, still in registers
+                else f = G[q][c * CH + j];  // the first step's f^{n+1} (departure in FP32), still in registers
+                dq[q] = f;
+                if constexpr (DEL) f = (q == 0 ? T(Md[j]) : T(maxw_b(Md[j], dkb))) + f;
                 fq[q] = f;
                 // the right difference f[L+1] - f[L] is the next lane's left one, bit for bit
                 const T d = f - shr1(f);
@@ -378,7 +470,7 @@ KBG_D void sweep1d(const Solve1D& S, const WorkItem& wi, int par, int lane, int 
             }
             // transport update G = f + (ff_left - ff_right) / dx
 #pragma unroll
-            for (int q = 0; q < NF; q++) G[q][c * CH + j] = fq[q] + (ffq[q] - shl1(ffq[q])) * rdx;
+            for (int q = 0; q < NF; q++) G[q][c * CH + j] = (DEL ? dq[q] : fq[q]) + (ffq[q] - shl1(ffq[q])) * rdx;
         });
 #pragma unroll
         for (int m = 0; m < NR; m++) acc[m] += double(part[m]);
@@ -418,8 +510,8 @@ KBG_D void sweep1d(const Solve1D& S, const WorkItem& wi, int par, int lane, int 
 
         // ---- wavefront 0 adds the wavefronts' sums (fixed order) and runs the cells' step! chain;
         // the others take its coefficients from LDS
-        T An, nl2, Un, inv, rTi, kbn, pre = 0, Ao = 0, nlo2 = 0, Uo = 0, lo2 = 0, kbo = 0;
-        T* coef = (T*)(lds + (size_t)NW * NR * kSweepLanes);  // [12][64]
+        T An, A1, nl2, Un, inv, rTi, kbn, pre = 0, Ao = 0, nlo2 = 0, Uo = 0, lo2 = 0, kbo = 0;
+        T* coef = (T*)(lds + (size_t)NW * NR * kSweepLanes);  // [13][64]
         if (NW > 1) {
 #pragma unroll
             for (int m = 0; m < NR; m++) lds[(wave * NR + m) * kSweepLanes + lane] = acc[m];
@@ -444,7 +536,7 @@ KBG_D void sweep1d(const Solve1D& S, const WorkItem& wi, int par, int lane, int 
                 for (int m = 0; m < 3; m++) wo[m] = win[(size_t)kc * 3 + m];
                 if (SHAK) {
 #pragma unroll
-                    for (int m = 0; m < 3; m++) po[m] = S.prim[(size_t)kc * 3 + m];
+                    for (int m = 0; m < 3; m++) po[m] = pq[m];
                 }
             } else {
 #pragma unroll
@@ -483,12 +575,14 @@ KBG_D void sweep1d(const Solve1D& S, const WorkItem& wi, int par, int lane, int 
 
             // the cell's step! coefficients (FP64 chain, lane-parallel)
             conserve_prim<3>(wn, S.gam, pn);
+            if (LAST ? upd : upd_mid) flag_health(S.health, prim_health<3>(pn));
             const T rT = relax_ratio<T>(pn[0], pn[2], S.mu, S.om, dt);
             inv = T(1) / (T(1) + rT);
             rTi = rT * inv;
             // the relaxation weight rT / (1 + rT) is folded into the equilibrium amplitude (and the
             // Shakhov prefactor below), one multiply per velocity less in pass 2
-            An = T(pn[0] * sqrt(pn[2] / kPi)) * rTi;
+            A1 = T(pn[0] * sqrt(pn[2] / kPi));
+            An = A1 * rTi;
             nl2 = T(-pn[2] * kLog2e);
             Un = T(pn[1]);
             kbn = NF == 2 ? T(K / (2.0 * pn[2])) : T(0);  // the b distribution's factor only
@@ -541,9 +635,9 @@ KBG_D void sweep1d(const Solve1D& S, const WorkItem& wi, int par, int lane, int 
                 if (lane < 12) rrow[lane] = y;
             }
             if (NW > 1) {
-                const T cv[12] = {An, nl2, Un, inv, rTi, kbn, pre, Ao, nlo2, Uo, lo2, kbo};
+                const T cv[13] = {An, nl2, Un, inv, rTi, kbn, pre, Ao, nlo2, Uo, lo2, kbo, A1};
 #pragma unroll
-                for (int m = 0; m < 12; m++) coef[m * kSweepLanes + lane] = cv[m];
+                for (int m = 0; m < 13; m++) coef[m * kSweepLanes + lane] = cv[m];
             }
         }
         if (NW > 1) {
@@ -555,6 +649,7 @@ KBG_D void sweep1d(const Solve1D& S, const WorkItem& wi, int par, int lane, int 
                 inv = coef[3 * kSweepLanes + lane];
                 rTi = coef[4 * kSweepLanes + lane];
                 kbn = coef[5 * kSweepLanes + lane];
+                A1 = coef[12 * kSweepLanes + lane];
                 if (SHAK) {
                     pre = coef[6 * kSweepLanes + lane];
                     Ao = coef[7 * kSweepLanes + lane];
@@ -599,6 +694,29 @@ KBG_D void sweep1d(const Solve1D& S, const WorkItem& wi, int par, int lane, int 
             if (vb >= vend) return;
             VC& ch = vt[vb / CH];
             T fn[NF][CH], Mn[CH], Mo[CH];
+            if constexpr (DEL) {
+                // departure form: d^{n+1} = f^{n+1} - M~^{n+1} = (M~^n - M~^{n+1} + G) / (1 + rT) + S rT / (1 + rT),
+                // G = d^n + (ff_left - ff_right) / dx; the Shakhov term from the old M~ times its prefactor
+                float Mnd[CH], Mod[CH];
+                maxw8(ch, A1, nl2, Un, Mnd);
+                maxw8(ch, dA, dnl, dU, Mod);
+                static_for<CH>([&](auto jc) {
+                    constexpr int j = jc;
+                    T sh0 = T(0), sh1 = T(0);
+                    if constexpr (SHAK) {
+                        const T co = ch.u[j] - Uo, c2 = co * co;
+                        const T t = co * (Mod[j] * pre);
+                        if constexpr (NF == 1) {
+                            sh0 = t * (lo2 * c2 - T(5));
+                        } else {
+                            sh0 = t * (lo2 * c2 + T(K) - T(5));
+                            sh1 = (t * kbo) * (lo2 * c2 + T(K) - T(3));
+                        }
+                    }
+                    fn[0][j] = ((Mod[j] - Mnd[j]) + G[0][c * CH + j]) * inv + sh0;
+                    if constexpr (NF == 2) fn[1][j] = ((maxw_b(Mod[j], dkb) - maxw_b(Mnd[j], kbn)) + G[1][c * CH + j]) * inv + sh1;
+                });
+            } else {
             maxw(ch, An, nl2, Un, Mn);
             if constexpr (SHAK) maxw(ch, Ao, nlo2, Uo, Mo);
             static_for<CH>([&](auto jc) {
@@ -618,6 +736,7 @@ KBG_D void sweep1d(const Solve1D& S, const WorkItem& wi, int par, int lane, int 
                 fn[0][j] = G[0][c * CH + j] * inv + MH;  // MH, MB carry the weight rT / (1 + rT)
                 if constexpr (NF == 2) fn[1][j] = G[1][c * CH + j] * inv + MB;
             });
+            }
             if constexpr (LAST) {
                 if (upd) {
                     if (vb + CH <= vend) {
@@ -657,7 +776,13 @@ KBG_D void sweep1d(const Solve1D& S, const WorkItem& wi, int par, int lane, int 
                 }
             }
         });
-
+        if constexpr (DEL && !LAST) {
+            // the second step's departures are from M~(prim^{n+1}) where the first step updated the cell
+            dA = upd_mid ? float(A1) : dA;
+            dnl = upd_mid ? float(nl2) : dnl;
+            dU = upd_mid ? float(Un) : dU;
+            dkb = upd_mid ? (NF == 2 ? float(kbn) : dkb) : dkb;
+        }
     });
 }
 
@@ -675,7 +800,7 @@ KBG_D void sweep_lim(const Solve1D& S, const WorkItem& wi, int par, int lane, in
 
 // LDS of one item (doubles): the wavefronts' moment sums and wavefront 0's cell coefficients
 template <class T, int NF, int FLUX> constexpr size_t sweep_lds_item(int nw) {
-    return nw > 1 ? (size_t)nw * SweepCfg<T, NF, FLUX>::NR * kSweepLanes + (12 * kSweepLanes * sizeof(T) + 7) / 8 : 0;
+    return nw > 1 ? (size_t)nw * SweepCfg<T, NF, FLUX>::NR * kSweepLanes + (13 * kSweepLanes * sizeof(T) + 7) / 8 : 0;
 }
 
 // occupancy target: 4 wavefronts per SIMD (<= 128 VGPRs) for the FP32 one-distribution sweeps,

@@ -27,6 +27,7 @@ struct Solve1D {
     int wallL, wallR;         // Maxwell wall at face 1 / nx+1 (2F only)
     int cur;                  // index of the current f / w buffer
     int swept;                // f is cell-fastest and stepped by k_sweep1d
+    int delta;                // FP32 swept: f buffers hold f - M~(prim) while the fused sweep owns them
     long np;                  // padded cells per velocity row (swept layout)
     long fcs, fqs, fvs;       // strides of f: cell, distribution, velocity
     double K, gam, mu, om, Pr, dt;
@@ -46,6 +47,7 @@ struct Solve1D {
     void* bff;                                     // [2][nf][nv]
     const double* bcprim;                          // [2][nw]  wall primitive vectors
     const void* vtab;                              // [nv + 8] per-velocity constants (swept path)
+    int* health;                                   // failure flags (kbg_device.h prim_health)
 };
 
 struct Solve2D {
@@ -74,6 +76,7 @@ struct Solve2D {
     double* xcoef;                                 // [(nx+1)*ny][kFaceCoef] KCU / UGKS face chain records
     double* ycoef;                                 // [nx*(ny+1)][kFaceCoef]
     double* rcell;                                 // [nx*ny][16] residual rows
+    int* health;                                   // failure flags (kbg_device.h prim_health)
 };
 
 // doubles per face-coefficient record (a UgksFace2 or a KcuFace<4>, padded)
@@ -121,6 +124,10 @@ constexpr int sweep_nw(bool fp64, int nf, int nv) {
 }
 // velocities per wavefront of an item (a multiple of 8, at most sweep_vw)
 constexpr int sweep_item_vw(int nv, int nw) { return ((nv + nw - 1) / nw + 7) / 8 * 8; }
+// Conversions of an FP32 swept solve's f buffer `buf` between its stored departure f - M~(prim) and f:
+// mode 0: FP64 staging (the swept layout) -> departure; 1: departure -> FP64 staging (exact, f = M~ + d);
+// 2: departure -> f in place (FP32); 3: f -> departure in place
+hipError_t launch_delta1d(const Solve1D& S, const Solve1D* d_S, double* stage, int mode, int buf, hipStream_t st);
 hipError_t launch_recon1d(bool fp64, const Solve1D& S, const Solve1D* d_S, hipStream_t st);
 hipError_t launch_evolve1d(bool fp64, const Solve1D& S, const Solve1D* d_S, hipStream_t st);
 hipError_t launch_wall1d(bool fp64, const Solve1D& S, const Solve1D* d_S, int fused, int par, hipStream_t st);

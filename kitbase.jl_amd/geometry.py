@@ -108,9 +108,10 @@ def global_frame(w: np.ndarray, c: float, s: float) -> np.ndarray:
     return np.array([w[0], w[1] * c - w[2] * s, w[1] * s + w[2] * c, w[3]])
 
 
-def face_geometry_2d(ps: PSpace2D):
+def face_geometry_2d(ps: PSpace2D, nf: int = 2):
     """Per-face normals and lengths the 2D evolve! uses (solver_evolution.jl:643-674,
-    evolve_boundary.jl:71-175).
+    evolve_boundary.jl:71-175; 2D1F2V walls solver_evolution.jl:500-604 take the edge length
+    ps.areas instead of dy / dx).
 
     Returns xgeo (ny, nx+1, 3) and ygeo (ny+1, nx, 3) with [n1, n2, len]:
       interior x-face (i, j), i = 2..nx: n = ps.n[i-1, j, 2], len = ps.areas[i-1, j, 2]
@@ -134,4 +135,9 @@ def face_geometry_2d(ps: PSpace2D):
     yg[0, :, 2] = ps.dx[0, :]
     yg[ny, :, 0:2] = ps.n[ny - 1, :, 2, :]
     yg[ny, :, 2] = ps.dx[ny - 1, :]
+    if nf == 1:
+        xg[:, 0, 2] = ps.areas[:, 0, 3]
+        xg[:, nx, 2] = ps.areas[:, nx - 1, 1]
+        yg[0, :, 2] = ps.areas[0, :, 0]
+        yg[ny, :, 2] = ps.areas[ny - 1, :, 2]
     return np.ascontiguousarray(xg), np.ascontiguousarray(yg)

@@ -10,7 +10,11 @@ Requests arrive over time (Poisson arrivals in the benchmark).  The loop
   * downloads a finished request's state, frees it and records its time to solution (arrival to
     completion).
 Each request's arithmetic is the same alone or in any mix (solves never share data; their sums have
-a fixed order), so results do not depend on what else is being served.
+a fixed order), so results do not depend on what else is being served.  The FP32 sweep advances a
+1D solve two steps per pass where it can, and a two-step pass may round differently (by an ulp, in
+rare FP32 Shakhov cases) from two one-step passes; so requests advance in step pairs counted from
+their own first step, as a solo kbg_batch_step does: a chunk has an even number of steps, and a
+request down to its last odd step takes it in a chunk of its own.
 
 The loop talks to the device through a backend object (DeviceBackend here; tests use a fake one).
 """
@@ -183,13 +187,19 @@ class Server:
                     self.sleep(max(0.0, pending[0].arrival - now()))
                 continue
             k = min(self.chunk, min(l.left for l in live))
+            if k == 1 and self.chunk > 1:
+                batch = [l for l in live if l.left == 1]  # last odd steps, on their own
+            else:
+                if k > 1:
+                    k -= k & 1
+                batch = live
             h0 = self.clock()
             if self.overlap:
                 err = []
 
                 def run():
                     try:
-                        self.backend.step([l.handle for l in live], [l.dt for l in live], k)
+                        self.backend.step([l.handle for l in batch], [l.dt for l in batch], k)
                     except BaseException as e:  # re-raised on the calling thread
                         err.append(e)
 
@@ -204,10 +214,10 @@ class Server:
                 if err:
                     raise err[0]
             else:
-                self.backend.step([l.handle for l in live], [l.dt for l in live], k)
+                self.backend.step([l.handle for l in batch], [l.dt for l in batch], k)
             stats.step_s += self.clock() - h0
             stats.chunks += 1
-            for l in live:
+            for l in batch:
                 l.left -= k
                 stats.updates += float(k) * l.req.points
             for l in [l for l in live if l.left == 0]:

@@ -201,6 +201,7 @@ def ib_cavity(set: Setup, ps: PSpace2D, vs: VSpace2D, gas: Gas, Um=0.15, Vm=0.0,
     h = maxwellian_2v(vs.u, vs.v, prim[0], prim[1], prim[2], prim[3])
     b = h * gas.K / 2.0 / prim[-1]
     p = dict(y1=ps.y1, w=w, prim=prim, primU=primU, h=h, b=b)
+    one_f = set.space == "2d1f2v"  # cfg_cavity.jl:37-40: f = maxwellian(u, v, prim) alone
 
     def fw(*_):
         return w
@@ -211,7 +212,7 @@ def ib_cavity(set: Setup, ps: PSpace2D, vs: VSpace2D, gas: Gas, Um=0.15, Vm=0.0,
         return primU if top else prim
 
     def ff(*_):
-        return h, b
+        return h if one_f else (h, b)
 
     return IB(fw, ff, bc, p)
 
@@ -337,9 +338,9 @@ def init_fvm(ks: SolverSet) -> tuple[ControlVolumes, Interfaces]:
         face = Interfaces(fwi, np.zeros((ps.nx + 1, nf, nv)))
         return ctr, face
     ps2: PSpace2D = ks.ps
-    nx, ny = ps2.nx, ps2.ny
+    nx, ny, nf = ps2.nx, ps2.ny, ks.nf
     w = np.empty((ny, nx, 4))
-    f = np.empty((ny, nx, 2, nv))
+    f = np.empty((ny, nx, nf, nv))
     seen2: dict[bytes, tuple[int, int]] = {}  # piecewise-constant initial states: evaluate each once
     for j in range(ny):
         for i in range(nx):
@@ -349,12 +350,16 @@ def init_fvm(ks: SolverSet) -> tuple[ControlVolumes, Interfaces]:
                 f[j, i] = f[seen2[key]]
                 continue
             seen2[key] = (j, i)
-            f[j, i, 0], f[j, i, 1] = ks.ib.ff(ps2.x[j, i], ps2.y[j, i])
+            if nf == 1:
+                f[j, i, 0] = ks.ib.ff(ps2.x[j, i], ps2.y[j, i])
+            else:
+                f[j, i, 0], f[j, i, 1] = ks.ib.ff(ps2.x[j, i], ps2.y[j, i])
     prim = conserve_prim(w, g)
-    ctr = ControlVolumes(w, prim, f, np.zeros((ny, nx, 2, 4)), np.zeros((ny, nx, 2, 2, nv)))
-    w0 = ks.ib.fw(ps2.x[0, 0], ps2.y[0, 0])
-    face = Interfaces(np.tile(w0, (ny, nx + 1, 1)), np.zeros((ny, nx + 1, 2, nv)),
-                      np.tile(w0, (ny + 1, nx, 1)), np.zeros((ny + 1, nx, 2, nv)))
+    ctr = ControlVolumes(w, prim, f, np.zeros((ny, nx, 2, 4)), np.zeros((ny, nx, 2, nf, nv)))
+    # Interface(W, H, B, ND) keeps fw = deepcopy(W) (struct_face.jl:96); Interface(W, F, ND) starts at 0
+    w0 = ks.ib.fw(ps2.x[0, 0], ps2.y[0, 0]) if nf == 2 else np.zeros(4)
+    face = Interfaces(np.tile(w0, (ny, nx + 1, 1)), np.zeros((ny, nx + 1, nf, nv)),
+                      np.tile(w0, (ny + 1, nx, 1)), np.zeros((ny + 1, nx, nf, nv)))
     return ctr, face
 
 
@@ -515,7 +520,7 @@ def make_desc(ks: SolverSet, precision: str = "fp64") -> tuple[_lib.KbgDesc, lis
         d.nx, d.ny, d.ng = ps.nx, ps.ny, 0
         d.dx = arr(ps.dx)
         d.dy = arr(ps.dy)
-        xg, yg = face_geometry_2d(ps)
+        xg, yg = face_geometry_2d(ps, ks.nf)
         d.xgeo, d.ygeo = arr(xg), arr(yg)
         xw, yw = wall_prims_2d(ks)
         d.xwall, d.ywall = arr(xw), arr(yw)
@@ -593,6 +598,13 @@ class DeviceSolve:
         dt = C.c_double()
         _lib.check(self.lib.kbg_timestep(self.handle, ks.set.cfl, ks.vs.u1, v1, simTime, ks.set.maxTime, C.byref(dt)))
         return dt.value
+
+    def health(self, clear: bool = False) -> int:
+        """The solve's failure flags (kbg_solve_health): 0, or KBG_HEALTH_NONFINITE | _RHO | _LAMBDA
+        bits raised by a cell update whose new primitive variables went bad."""
+        flags = C.c_int(0)
+        _lib.check(self.lib.kbg_solve_health(self.handle, C.byref(flags), 1 if clear else 0))
+        return flags.value
 
     def step(self, dt: float, nsteps: int = 1, residual: np.ndarray | None = None):
         r = None

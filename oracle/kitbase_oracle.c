@@ -586,6 +586,29 @@ void orc_flux_kfvs_2f2v(double *fw, double *fh, double *fb, const double *hL, co
     fw[3] = 0.5 * (s[3] + s[4]);
 }
 
+/* 2D1F2V   flux_kfvs.jl:615-647 (flux_conserve! theory_moments.jl:403-407) */
+void orc_flux_kfvs_1f2v(double *fw, double *ff, const double *fL, const double *fR, const double *u,
+                        const double *v, const double *w, int n, double dt, double len,
+                        const double *sfL, const double *sfR) {
+    double s[4] = {0, 0, 0, 0};
+    for (int i = 0; i < n; i++) {
+        double d = orc_heaviside(u[i]);
+        double f = fL[i] * d + fR[i] * (1.0 - d);
+        double sf = (sfL ? sfL[i] : 0.0) * d + (sfR ? sfR[i] : 0.0) * (1.0 - d);
+        ff[i] = (dt * u[i] * f - 0.5 * (dt * dt) * (u[i] * u[i]) * sf) * len;
+    }
+    for (int i = 0; i < n; i++) {
+        s[0] += w[i] * ff[i];
+        s[1] += u[i] * w[i] * ff[i];
+        s[2] += v[i] * w[i] * ff[i];
+        s[3] += ((u[i] * u[i]) + (v[i] * v[i])) * w[i] * ff[i];
+    }
+    fw[0] = s[0];
+    fw[1] = s[1];
+    fw[2] = s[2];
+    fw[3] = 0.5 * s[3];
+}
+
 /* ------------------------------------------------------------------------------------------ */
 /* KCU   src/Flux/flux_kcu.jl                                                                   */
 /* ------------------------------------------------------------------------------------------ */
@@ -757,6 +780,44 @@ void orc_flux_kcu_2f2v(double *fw, double *fh, double *fb, const double *wL, con
     fw[1] += Mt2 * s[1] * len;
     fw[2] += Mt2 * s[2] * len;
     fw[3] += Mt2 * 0.5 * (s[3] + s[4]) * len;
+}
+
+/* 2D1F2V   flux_kcu.jl:828-891 */
+void orc_flux_kcu_1f2v(double *fw, double *ff, const double *wL, const double *fL, const double *wR,
+                       const double *fR, const double *u, const double *v, const double *w, int n,
+                       double K, double gam, double mu, double om, double Pr, double dt, double len) {
+    (void)Pr;
+    double primL[4], primR[4], Muv1[4], Muv2[4], ww[4], prim[4], Muv[4];
+    orc_gm g1, g2, g;
+    orc_conserve_prim(wL, 4, gam, primL);
+    orc_conserve_prim(wR, 4, gam, primR);
+    orc_gauss_moments(primL, 4, K, &g1);
+    orc_moments_conserve_2d(g1.MuL, g1.Mv, g1.Mxi, 0, 0, 0, Muv1);
+    orc_gauss_moments(primR, 4, K, &g2);
+    orc_moments_conserve_2d(g2.MuR, g2.Mv, g2.Mxi, 0, 0, 0, Muv2);
+    for (int m = 0; m < 4; m++) ww[m] = primL[0] * Muv1[m] + primR[0] * Muv2[m];
+    orc_conserve_prim(ww, 4, gam, prim);
+    double tau = orc_vhs_collision_time(prim[0], prim[3], mu, om) + kcu_tau_extra(primL, primR, 4, dt);
+    double Mt2 = tau * (1.0 - exp(-dt / tau)), Mt1 = dt - Mt2;
+    orc_gauss_moments(prim, 4, K, &g);
+    orc_moments_conserve_2d(g.Mu, g.Mv, g.Mxi, 1, 0, 0, Muv);
+    for (int m = 0; m < 4; m++) fw[m] = Mt1 * prim[0] * Muv[m] * len;
+    double s[4] = {0, 0, 0, 0};
+    for (int i = 0; i < n; i++) {
+        double d = orc_heaviside(u[i]);
+        double f = fL[i] * d + fR[i] * (1.0 - d);
+        double cu = u[i] - prim[1], cv = v[i] - prim[2];
+        double g0 = prim[0] * (prim[3] / PI_) * exp(-prim[3] * (cu * cu + cv * cv));
+        s[0] += w[i] * u[i] * f;
+        s[1] += w[i] * (u[i] * u[i]) * f;
+        s[2] += w[i] * v[i] * u[i] * f;
+        s[3] += w[i] * u[i] * ((u[i] * u[i]) + (v[i] * v[i])) * f;
+        ff[i] = (Mt1 * u[i] * g0 + Mt2 * u[i] * f) * len;
+    }
+    fw[0] += Mt2 * s[0] * len;
+    fw[1] += Mt2 * s[1] * len;
+    fw[2] += Mt2 * s[2] * len;
+    fw[3] += Mt2 * 0.5 * s[3] * len;
 }
 
 /* ------------------------------------------------------------------------------------------ */
@@ -997,6 +1058,37 @@ void orc_flux_boundary_maxwell_2f2v(double *fw, double *fh, double *fb, const do
     fw[3] = (0.5 * s[3] + 0.5 * s[4]) * len * dt;
 }
 
+/* 2D1F2V   flux_boundary.jl:160-183 (bc in the local frame, velocities u=vn, v=vt) */
+void orc_flux_boundary_maxwell_1f2v(double *fw, double *fh, const double *bc, const double *h,
+                                    const double *u, const double *v, const double *w, int n,
+                                    double dt, double len, int rot) {
+    double SF = 0, SG = 0;
+    for (int i = 0; i < n; i++) {
+        double d = orc_heaviside(u[i] * rot);
+        double cu = u[i] - bc[1], cv = v[i] - bc[2];
+        SF += w[i] * u[i] * h[i] * (1.0 - d);
+        SG += w[i] * u[i] * exp(-bc[3] * (cu * cu + cv * cv)) * d;
+    }
+    SG *= (bc[3] / PI_);
+    double prim[4] = {-SF / SG, bc[1], bc[2], bc[3]};
+    double s[4] = {0, 0, 0, 0};
+    for (int i = 0; i < n; i++) {
+        double d = orc_heaviside(u[i] * rot);
+        double cu = u[i] - prim[1], cv = v[i] - prim[2];
+        double H = prim[0] * (prim[3] / PI_) * exp(-prim[3] * (cu * cu + cv * cv));
+        double hW = H * d + h[i] * (1.0 - d);
+        s[0] += w[i] * u[i] * hW;
+        s[1] += w[i] * (u[i] * u[i]) * hW;
+        s[2] += w[i] * v[i] * (hW * u[i]);
+        s[3] += w[i] * u[i] * (hW * ((u[i] * u[i]) + (v[i] * v[i])));
+        fh[i] = u[i] * hW * len * dt;
+    }
+    fw[0] = s[0] * len * dt;
+    fw[1] = s[1] * len * dt;
+    fw[2] = s[2] * len * dt;
+    fw[3] = 0.5 * s[3] * len * dt;
+}
+
 /* ------------------------------------------------------------------------------------------ */
 /* step!   src/Solver/step_1f.jl, step_2f.jl                                                    */
 /* ------------------------------------------------------------------------------------------ */
@@ -1105,6 +1197,32 @@ int orc_step_1f3v(double *w, double *prim, double *f, const double *fwL, const d
         double cu = u[i] - prim[1], cv = v[i] - prim[2], cw = wv[i] - prim[3];
         double M = prim[0] * sqrt(l3 * l3 * l3) * exp(-prim[4] * (cu * cu + cv * cv + cw * cw));
         f[i] = (f[i] + (ffL[i] - ffR[i]) / dx + dt / tau * M) / (1.0 + dt / tau);
+    }
+    return 0;
+}
+
+/* 2D1F2V   step_1f.jl:164-221.  The Shakhov branch there reads heat_flux(h, b, ...) with b undefined
+ * (step_1f.jl:194), so KitBase throws; this returns -1 for it and steps BGK otherwise. */
+int orc_step_1f2v(double *w, double *prim, double *f, const double *fwL, const double *ffL,
+                  const double *fwR, const double *ffR, const double *fwD, const double *ffD,
+                  const double *fwU, const double *ffU, const double *u, const double *v,
+                  const double *weights, int n, double gam, double mu, double om, double Pr,
+                  double ds, double dt, double *RES, double *AVG, int coll) {
+    (void)weights;
+    (void)Pr;
+    if (coll == ORC_SHAKHOV) return -1;
+    double w_old[4] = {w[0], w[1], w[2], w[3]};
+    for (int m = 0; m < 4; m++) w[m] += (fwL[m] - fwR[m] + fwD[m] - fwU[m]) / ds;
+    orc_conserve_prim(w, 4, gam, prim);
+    for (int m = 0; m < 4; m++) {
+        RES[m] += (w[m] - w_old[m]) * (w[m] - w_old[m]);
+        AVG[m] += fabs(w[m]);
+    }
+    double tau = orc_vhs_collision_time(prim[0], prim[3], mu, om);
+    for (int i = 0; i < n; i++) {
+        double cu = u[i] - prim[1], cv = v[i] - prim[2];
+        double M = prim[0] * (prim[3] / PI_) * exp(-prim[3] * (cu * cu + cv * cv));
+        f[i] = (f[i] + (ffL[i] - ffR[i] + ffD[i] - ffU[i]) / ds + dt / tau * M) / (1.0 + dt / tau);
     }
     return 0;
 }
@@ -1328,13 +1446,17 @@ int orc_run_1d(orc_solver1d *S, double dt, int nsteps, double *res) {
 
 /* slopes: sw[cell][dir][4], sf[cell][dir][2][nv] */
 static double *SW2(orc_solver2d *S, size_t c, int dir) { return S->sw + c * 8 + (size_t)dir * 4; }
-static double *SF2(orc_solver2d *S, size_t c, int dir) { return S->sf + c * 4 * (size_t)S->nv + (size_t)dir * 2 * S->nv; }
+static int nf_2d(const orc_solver2d *S) { return S->nf == 1 ? 1 : 2; }
+static double *SF2(orc_solver2d *S, size_t c, int dir) {
+    const size_t NF = (size_t)nf_2d(S) * S->nv;
+    return S->sf + c * 2 * NF + (size_t)dir * NF;
+}
 
 /* reconstruct!   src/Solver/solver_reconstruction.jl:475-620 (reconstruct2! reconstruction.jl:48-50
  * on the boundary rows, reconstruct3! inside) */
 int orc_reconstruct_2d(orc_solver2d *S) {
     if (S->interp_order == 1) return 0;
-    const int nx = S->nx, ny = S->ny, nv = S->nv, NF = 2 * nv;
+    const int nx = S->nx, ny = S->ny, nv = S->nv, NF = nf_2d(S) * nv;
     /* boundary x */
 #pragma omp parallel for schedule(static)
     for (int j = 1; j <= ny; j++) {
@@ -1397,7 +1519,35 @@ int orc_reconstruct_2d(orc_solver2d *S) {
 static void face_2d(orc_solver2d *S, size_t cL, size_t cR, int dir, double dxL, double dxR,
                     const double *n, double len, double dt, double *fw, double *fh, double *fb,
                     double *buf) {
-    const int nv = S->nv, NF = 2 * nv;
+    const int nv = S->nv, NF = nf_2d(S) * nv;
+    if (nf_2d(S) == 1) {
+        /* 2D1F2V: the 1F cell-level wrappers flux_kfvs.jl:65-93, flux_kcu.jl:104-147 */
+        double *fL = buf, *fR = buf + nv, *vn = buf + 2 * nv, *vt = buf + 3 * nv;
+        const double *sL = SF2(S, cL, dir), *sR = SF2(S, cR, dir);
+        const double *f0 = S->f + cL * NF, *f1 = S->f + cR * NF;
+        for (int q = 0; q < nv; q++) {
+            fL[q] = f0[q] + sL[q] * dxL;
+            fR[q] = f1[q] - sR[q] * dxR;
+            vn[q] = S->u[q] * n[0] + S->v[q] * n[1];
+            vt[q] = S->v[q] * n[0] - S->u[q] * n[1];
+        }
+        double fwl[4];
+        if (S->flux == ORC_KFVS) {
+            orc_flux_kfvs_1f2v(fwl, fh, fL, fR, vn, vt, S->weights, nv, dt, len, sL, sR);
+        } else {
+            double wl[4], wr[4], WL[4], WR[4];
+            const double *swL = SW2(S, cL, dir), *swR = SW2(S, cR, dir);
+            for (int m = 0; m < 4; m++) {
+                wl[m] = S->w[cL * 4 + m] + dxL * swL[m];
+                wr[m] = S->w[cR * 4 + m] - dxR * swR[m];
+            }
+            orc_local_frame(wl, 4, n[0], n[1], WL);
+            orc_local_frame(wr, 4, n[0], n[1], WR);
+            orc_flux_kcu_1f2v(fwl, fh, WL, fL, WR, fR, vn, vt, S->weights, nv, S->K, S->gam, S->mu, S->om, S->Pr, dt, len);
+        }
+        orc_global_frame(fwl, 4, n[0], n[1], fw);
+        return;
+    }
     double *hL = buf, *bL = buf + nv, *hR = buf + 2 * nv, *bR = buf + 3 * nv, *vn = buf + 4 * nv, *vt = buf + 5 * nv;
     const double *sL = SF2(S, cL, dir), *sR = SF2(S, cR, dir);
     const double *fL = S->f + cL * NF, *fR = S->f + cR * NF;
@@ -1432,7 +1582,7 @@ static void face_2d(orc_solver2d *S, size_t cL, size_t cR, int dir, double dxL, 
 /* a wall face: evolve_boundary.jl:71-175 -> flux_boundary.jl:190-220 */
 static void wall_2d(orc_solver2d *S, size_t c, const double *n, double len, const double *bcprim,
                     int rot, double dt, double *fw, double *fh, double *fb, double *buf) {
-    const int nv = S->nv, NF = 2 * nv;
+    const int nv = S->nv, NF = nf_2d(S) * nv;
     double *vn = buf, *vt = buf + nv;
     for (int q = 0; q < nv; q++) {
         vn[q] = S->u[q] * n[0] + S->v[q] * n[1];
@@ -1440,15 +1590,19 @@ static void wall_2d(orc_solver2d *S, size_t c, const double *n, double len, cons
     }
     double bcl[4], fwl[4];
     orc_local_frame(bcprim, 4, n[0], n[1], bcl);
-    orc_flux_boundary_maxwell_2f2v(fwl, fh, fb, bcl, S->f + c * NF, S->f + c * NF + nv, vn, vt, S->weights, nv, S->K, dt, len, rot);
+    if (nf_2d(S) == 1) /* the inline 2D1F walls of solver_evolution.jl:500-604 */
+        orc_flux_boundary_maxwell_1f2v(fwl, fh, bcl, S->f + c * NF, vn, vt, S->weights, nv, dt, len, rot);
+    else
+        orc_flux_boundary_maxwell_2f2v(fwl, fh, fb, bcl, S->f + c * NF, S->f + c * NF + nv, vn, vt, S->weights, nv, S->K, dt, len, rot);
     orc_global_frame(fwl, 4, n[0], n[1], fw);
 }
 
 /* evolve!   src/Solver/solver_evolution.jl:608-679 (no ghost layers) + evolve_boundary.jl:52-178 */
 int orc_evolve_2d(orc_solver2d *S, double dt) {
-    const int nx = S->nx, ny = S->ny, nv = S->nv, NF = 2 * nv;
+    const int nx = S->nx, ny = S->ny, nv = S->nv, NF = nf_2d(S) * nv;
     for (int s = 0; s < 4; s++)
         if (S->bc[s] != ORC_BC_MAXWELL && S->bc[s] != ORC_BC_FIX) return -4;
+    if (nf_2d(S) == 1 && S->flux == ORC_UGKS) return -2; /* KitBase has no 2D1F2V UGKS kernel */
 #pragma omp parallel
     {
         double *buf = (double *)malloc(sizeof(double) * 6 * nv);
@@ -1503,15 +1657,21 @@ int orc_evolve_2d(orc_solver2d *S, double dt) {
     return 0;
 }
 
-static void step_cell_2d(orc_solver2d *S, int i, int j, double dt, double *RES, double *AVG) {
-    const int nv = S->nv, NF = 2 * nv;
+static int step_cell_2d(orc_solver2d *S, int i, int j, double dt, double *RES, double *AVG) {
+    const int nv = S->nv, NF = nf_2d(S) * nv;
     size_t c = C2(i, j), L = XF(i, j), R = XF(i + 1, j), D = YF(i, j), U = YF(i, j + 1);
     double *f = S->f + c * NF;
+    if (nf_2d(S) == 1)
+        return orc_step_1f2v(S->w + 4 * c, S->prim + 4 * c, f, S->xfw + 4 * L, S->xff + L * NF, S->xfw + 4 * R,
+                             S->xff + R * NF, S->yfw + 4 * D, S->yff + D * NF, S->yfw + 4 * U, S->yff + U * NF, S->u,
+                             S->v, S->weights, nv, S->gam, S->mu, S->om, S->Pr, S->dx[c] * S->dy[c], dt, RES, AVG,
+                             S->coll);
     orc_step_2f2v(S->w + 4 * c, S->prim + 4 * c, f, f + nv, S->xfw + 4 * L, S->xff + L * NF,
                   S->xff + L * NF + nv, S->xfw + 4 * R, S->xff + R * NF, S->xff + R * NF + nv,
                   S->yfw + 4 * D, S->yff + D * NF, S->yff + D * NF + nv, S->yfw + 4 * U,
                   S->yff + U * NF, S->yff + U * NF + nv, S->u, S->v, S->weights, nv, S->K, S->gam,
                   S->mu, S->om, S->Pr, S->dx[c] * S->dy[c], dt, RES, AVG, S->coll);
+    return 0;
 }
 
 /* update!   src/Solver/solver_update.jl:193-246, then update_boundary! update_boundary.jl:172-306
@@ -1519,6 +1679,7 @@ static void step_cell_2d(orc_solver2d *S, int i, int j, double dt, double *RES, 
 int orc_update_2d(orc_solver2d *S, double dt, double *res) {
     const int nx = S->nx, ny = S->ny;
     double sumRes[4] = {0}, sumAvg[4] = {0};
+    if (nf_2d(S) == 1 && S->coll == ORC_SHAKHOV) return -1; /* step_1f.jl:194 */
 #pragma omp parallel
     {
         double r[4] = {0}, a[4] = {0};

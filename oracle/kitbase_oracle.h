@@ -175,7 +175,24 @@ typedef struct {
     const double *yn, *ylen, *ywall; /* nx*(ny+1) */
     double *w, *prim, *f, *sw, *sf;
     double *xfw, *xff, *yfw, *yff;
+    int nf;                          /* distributions: 1 (2D1F2V, f) or 2 (2D2F2V, h and b) */
 } orc_solver2d;
+
+/* 2D1F2V kernels */
+void orc_flux_kfvs_1f2v(double *fw, double *ff, const double *fL, const double *fR, const double *u,
+                        const double *v, const double *w, int n, double dt, double len,
+                        const double *sfL, const double *sfR);
+void orc_flux_kcu_1f2v(double *fw, double *ff, const double *wL, const double *fL, const double *wR,
+                       const double *fR, const double *u, const double *v, const double *w, int n,
+                       double K, double gam, double mu, double om, double Pr, double dt, double len);
+void orc_flux_boundary_maxwell_1f2v(double *fw, double *fh, const double *bc, const double *h,
+                                    const double *u, const double *v, const double *w, int n,
+                                    double dt, double len, int rot);
+int orc_step_1f2v(double *w, double *prim, double *f, const double *fwL, const double *ffL,
+                  const double *fwR, const double *ffR, const double *fwD, const double *ffD,
+                  const double *fwU, const double *ffU, const double *u, const double *v,
+                  const double *weights, int n, double gam, double mu, double om, double Pr,
+                  double ds, double dt, double *RES, double *AVG, int coll);
 
 int orc_reconstruct_2d(orc_solver2d *S);                    /* solver_reconstruction.jl:475-620 */
 int orc_evolve_2d(orc_solver2d *S, double dt);              /* solver_evolution.jl:608-679, evolve_boundary.jl:52-178 */

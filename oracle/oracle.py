@@ -78,6 +78,7 @@ class Solver2DStruct(C.Structure):
         ("xn", _dp), ("xlen", _dp), ("xwall", _dp), ("yn", _dp), ("ylen", _dp), ("ywall", _dp),
         ("w", _dp), ("prim", _dp), ("f", _dp), ("sw", _dp), ("sf", _dp),
         ("xfw", _dp), ("xff", _dp), ("yfw", _dp), ("yff", _dp),
+        ("nf", C.c_int),
     ]
 
 
@@ -120,14 +121,14 @@ class OracleSolve:
             kb = sys.modules["kitbase_amd"]  # the host mirror supplies the face geometry
             S = Solver2DStruct()
             ps, vs = ks.ps, ks.vs
-            S.nx, S.ny, S.nv = ps.nx, ps.ny, vs.nv
+            S.nx, S.ny, S.nv, S.nf = ps.nx, ps.ny, vs.nv, ks.nf
             S.flux, S.coll, S.limiter, S.interp_order = FLUX[ks.set.flux], COLL[ks.set.collision], LIM[ks.set.limiter], ks.set.interpOrder
             for i in range(4):
                 S.bc[i] = BC[bcs[i]]
             S.K, S.gam, S.mu, S.om, S.Pr = g.K, g.gamma, g.mu_ref, g.omega, g.Pr
             S.u, S.v, S.weights = k(vs.u), k(vs.v), k(vs.weights)
             S.dx, S.dy = k(ps.dx), k(ps.dy)
-            xg, yg = kb.geometry.face_geometry_2d(ps)
+            xg, yg = kb.geometry.face_geometry_2d(ps, ks.nf)
             xw, yw = kb.solver.wall_prims_2d(ks)
             S.xn, S.xlen, S.xwall = k(xg[..., 0:2]), k(xg[..., 2]), k(xw)
             S.yn, S.ylen, S.ywall = k(yg[..., 0:2]), k(yg[..., 2]), k(yw)

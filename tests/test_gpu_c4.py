@@ -1,0 +1,95 @@
+"""The batched workload (512 independent 1D1F1V shock tubes from the committed seed: nx 64-2048,
+nv 32-256, Kn 1e-4..10, BGK / Shakhov) stepped together in FP32 by kbg_batch_step, against the FP64
+oracle on the same inputs, instance by instance, with the plain relative errors
+
+    L2:  ||x_gpu - x_oracle||_2 / ||x_oracle||_2        Linf:  max |x_gpu - x_oracle| / max |x_oracle|
+
+for x = rho, U, T (no other normalisation of U).  Tolerance 1e-4 (relative L2, the FP32 bar of the
+batched path); the largest L-infinity errors are printed and held to 3e-4."""
+import copy
+
+import numpy as np
+import pytest
+
+from conftest import kb, oracle
+
+pytestmark = pytest.mark.gpu
+
+L2_TOL, LINF_TOL = 1e-4, 3e-4
+
+
+def macro(prim):
+    p = np.asarray(prim)
+    return p[:, 0], p[:, 1], 1.0 / (2.0 * p[:, 2])
+
+
+def errors(pg, po):
+    out = {}
+    for name, a, b in zip(("rho", "U", "T"), macro(pg), macro(po)):
+        d = a - b
+        out[name + "_l2"] = float(np.linalg.norm(d) / np.linalg.norm(b))
+        out[name + "_linf"] = float(np.max(np.abs(d)) / np.max(np.abs(b)))
+    return out
+
+
+def _instances(idx):
+    inst = kb.configs.batch_instances(512)
+    out = []
+    for k in idx:
+        ks = kb.SolverSet(kb.configs.riemann_1d1f1v(inst[k]))
+        ctr, face = kb.init_fvm(ks)
+        out.append((k, ks, ctr, face, kb.timestep(ks, ctr)))
+    return out
+
+
+def _run_gpu(runs, nsteps_of, ctx):
+    """FP32 batch: every instance advances together; an instance leaves the batch at its own step count"""
+    devs = [kb.DeviceSolve(ks, copy.deepcopy(ctr), face, precision="fp32", ctx=ctx) for _, ks, ctr, face, _ in runs]
+    nts = np.array([nsteps_of(r) for r in runs])
+    done, live = 0, list(np.argsort(nts, kind="stable"))
+    while live:
+        m = int(min(nts[i] for i in live))
+        kb.batch_step([devs[i] for i in live], [runs[i][4] for i in live], m - done, ctx=ctx)
+        done = m
+        live = [i for i in live if nts[i] > done]
+    out = []
+    for d, (_, ks, ctr, face, _) in zip(devs, runs):
+        c = copy.deepcopy(ctr)
+        d.download(c)
+        d.close()
+        out.append(c.prim)
+    return out, nts
+
+
+def _check(runs, res, nts, label):
+    rows = []
+    for (k, ks, ctr, face, dt), pg, nt in zip(runs, res, nts):
+        c, f = copy.deepcopy(ctr), copy.deepcopy(face)
+        oracle.OracleSolve(ks, c, f).run(dt, int(nt))
+        e = errors(pg, c.prim)
+        rows.append((k, e, float(np.linalg.norm(c.prim[:, 1]) / np.sqrt(len(c.prim)))))
+    worst = {key: max(rows, key=lambda r: r[1][key]) for key in rows[0][1]}
+    for key, (k, e, nu) in worst.items():
+        print(f"{label}: worst {key} {e[key]:.3e} (instance {k}, rms U {nu:.3e})", flush=True)
+    bad = [(k, key, e[key]) for k, e, _ in rows for key in e
+           if e[key] > (L2_TOL if key.endswith("_l2") else LINF_TOL)]
+    assert not bad, bad
+
+
+def test_batch_all_instances_against_the_oracle_300_steps(gpu_ctx):
+    """All 512 instances at their full sizes, 300 steps of the batch (each instance its own CFL dt)."""
+    runs = _instances(range(512))
+    res, nts = _run_gpu(runs, lambda r: 300, gpu_ctx)
+    _check(runs, res, nts, "512 x 300 steps")
+
+
+# instances whose velocity field is weakest (equal pressures on both sides: U stays near 0, the
+# hardest case for a relative U error), and a spread of the others, run to t = 0.2
+_T02 = [321, 479, 175, 67, 354, 336, 376, 262, 178, 18, 0, 100, 255, 384, 420, 511]
+
+
+def test_batch_to_t_end_against_the_oracle(gpu_ctx):
+    """Instances run to KitBase's t_end = 0.2 (nt = floor(0.2 / dt) + 1 steps each, up to ~5500)."""
+    runs = _instances(_T02)
+    res, nts = _run_gpu(runs, lambda r: int(np.floor(r[1].set.maxTime / r[4])) + 1, gpu_ctx)
+    _check(runs, res, nts, "t = 0.2")

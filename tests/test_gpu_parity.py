@@ -470,7 +470,7 @@ _CK_CASES = {
 def test_checkpoint_resume_bitwise(gpu_ctx, case, tmp_path):
     """download (before write_jld, solver_main.jl:59-61) -> file -> upload into a new solve (as after
     initialize("*.jld2"), solver_init.jl:19-26) -> continue: the resumed solve matches the
-    uninterrupted one bit for bit."""
+    uninterrupted one bit for bit (the FP32 1D distributions to 1e-13, see below)."""
     cfgf, prec = _CK_CASES[case]
     ks = kb.SolverSet(cfgf())
     ctr, face = kb.init_fvm(ks)
@@ -490,7 +490,13 @@ def test_checkpoint_resume_bitwise(gpu_ctx, case, tmp_path):
     ca, cb = copy.deepcopy(ctr), copy.deepcopy(ctr)
     a.download(ca)
     b.download(cb)
-    assert np.array_equal(ca.f, cb.f) and np.array_equal(ca.w, cb.w) and np.array_equal(ca.prim, cb.prim)
+    assert np.array_equal(ca.w, cb.w) and np.array_equal(ca.prim, cb.prim)
+    if prec == "fp32" and ks.dim == 1:
+        # the FP32 sweep stores f - M~(prim); a checkpoint holds M~ + departure in FP64, which returns
+        # the departure bit for bit unless it lies 2^29 below f (then within one FP32 ulp of it)
+        assert np.allclose(ca.f, cb.f, rtol=1e-13, atol=0.0)
+    else:
+        assert np.array_equal(ca.f, cb.f)
     a.close()
     b.close()
 
@@ -549,6 +555,12 @@ _TWO_CASES = [
     (dict(nx=50, nu=24, vMeshType="rectangle"), "fp32"),                   # 1 wavefront
     (dict(nx=60, limiter="minmod", collision="shakhov"), "fp64"),
     (dict(nx=60, interpOrder=1), "fp32"),
+    # 16 velocities per wavefront (the serving backend's cap): 4 wavefronts for 33 velocities, the
+    # last one empty, the third holding one
+    (dict(nx=60, nu=33, vMeshType="rectangle", collision="shakhov", vw=16), "fp32"),
+    (dict(nx=60, nu=33, vMeshType="rectangle", collision="bgk", vw=16), "fp32"),
+    (dict(nx=60, nu=48, vMeshType="rectangle", collision="shakhov", vw=16), "fp32"),
+    (dict(nx=60, nu=33, vMeshType="rectangle", collision="shakhov"), "fp32"),
 ]
 
 
@@ -558,11 +570,15 @@ def test_two_steps_per_sweep_launch_bitwise(gpu_ctx, case, nsteps):
     """The fused sweep advancing two steps per pass (4-cell halo, f^{n+1} kept in registers) gives the
     same bits as one step per launch, for odd and even step counts and several segment lengths."""
     over, prec = _TWO_CASES[case]
+    over = dict(over)
+    vw = over.pop("vw", 0)
     ks = kb.SolverSet(kb.configs.sod_1d1f1v(**over))
     ctr, face = kb.init_fvm(ks)
     dt = kb.timestep(ks, ctr)
     one_ctx = kb.Context(0)
     one_ctx.set_sweep_steps(1)
+    one_ctx.set_sweep_vw(vw)
+    gpu_ctx.set_sweep_vw(vw)
     old = gpu_ctx.schedule()
     try:
         for seg in (56, 13):
@@ -584,3 +600,4 @@ def test_two_steps_per_sweep_launch_bitwise(gpu_ctx, case, nsteps):
             b.close()
     finally:
         gpu_ctx.set_schedule(*old)
+        gpu_ctx.set_sweep_vw(0)

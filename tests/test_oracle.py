@@ -945,3 +945,129 @@ def test_kcu_2d_analytic_face_moments_match_its_distribution_flux(mu):
     call("orc_flux_kcu_2f2v", fw, fh, fb, WL, hL, bL, WR, hR, bR, u, v, w, n, K, gam, mu, 0.72, 1.0, 2e-3, length)
     mom = _moments_2f2v(u, v, w, fh, fb)
     assert np.allclose(mom, fw, rtol=1e-8, atol=1e-12)
+
+
+# ---------------------------------------------------------------- 2D1F2V (one distribution, gamma = 2)
+
+def _max_2v(prim, u, v):
+    rho, U, V, lam = prim
+    return A(rho * (lam / np.pi) * np.exp(-lam * ((u - U) ** 2 + (v - V) ** 2)))
+
+
+@pytest.mark.parametrize("flux", ["kfvs", "kcu"])
+def test_1f2v_uniform_maxwellian_gives_euler_flux(flux):
+    """A uniform Maxwellian through a face: the Euler flux of a 2D monatomic gas (gamma = 2,
+    E = rho |U|^2 / 2 + p, p = rho / (2 lambda)), times dt and the face length."""
+    u, v, w = tensor_grid(96, -9.0, 9.0, 2)
+    gam = 2.0
+    prim = [1.1, 0.3, -0.2, 0.9]
+    f = _max_2v(prim, u, v)
+    n = u.size
+    fw = A(np.zeros(4)); ff = A(np.zeros(n)); z = A(np.zeros(n))
+    dt, length = 2e-3, 0.7
+    if flux == "kfvs":
+        call("orc_flux_kfvs_1f2v", fw, ff, f, f, u, v, w, n, dt, length, z, z)
+    else:
+        W = A(np.zeros(4)); call("orc_prim_conserve", A(prim), 4, gam, W)
+        call("orc_flux_kcu_1f2v", fw, ff, W, f, W, f, u, v, w, n, 0.0, gam, 1e-2, 0.72, 1.0, dt, length)
+    assert np.allclose(fw, dt * length * euler_flux_2d(prim, 0.0), rtol=1e-8, atol=1e-13)
+    # the face moments are the discrete moments of the distribution flux
+    mom = np.array([np.dot(w, ff), np.dot(w, u * ff), np.dot(w, v * ff), 0.5 * np.dot(w, (u * u + v * v) * ff)])
+    if flux == "kfvs":
+        assert np.allclose(mom, fw, rtol=1e-12)
+
+
+@pytest.mark.parametrize("mu", [1e-3, 1e-1])
+def test_kcu_1f2v_analytic_face_moments_match_its_distribution_flux(mu):
+    u, v, w = _aniso_grid(1201, 81, 13.0, 10.0)
+    gam = 2.0
+    pl, pr = [1.0, 0.25, 0.1, 0.8], [0.7, -0.15, -0.2, 1.1]
+    fL, fR = _max_2v(pl, u, v), _max_2v(pr, u, v)
+    WL, WR = A(np.zeros(4)), A(np.zeros(4))
+    call("orc_prim_conserve", A(pl), 4, gam, WL); call("orc_prim_conserve", A(pr), 4, gam, WR)
+    n, length = u.size, 0.6
+    fw = A(np.zeros(4)); ff = A(np.zeros(n))
+    call("orc_flux_kcu_1f2v", fw, ff, WL, fL, WR, fR, u, v, w, n, 0.0, gam, mu, 0.72, 1.0, 2e-3, length)
+    mom = np.array([np.dot(w, ff), np.dot(w, u * ff), np.dot(w, v * ff), 0.5 * np.dot(w, (u * u + v * v) * ff)])
+    assert np.allclose(mom, fw, rtol=1e-8, atol=1e-12)
+
+
+@pytest.mark.parametrize("rot", [1, -1])
+def test_maxwell_wall_1f2v_zero_mass_flux_and_pressure(rot):
+    u, v, w = tensor_grid(48, -8.0, 8.0, 2)
+    n = u.size
+    rng = np.random.default_rng(19)
+    f = A(rng.uniform(0.0, 1.0, n) * np.exp(-0.5 * (u * u + v * v)))
+    fw = A(np.zeros(4)); ff = A(np.zeros(n))
+    call("orc_flux_boundary_maxwell_1f2v", fw, ff, A([1.0, 0.0, 0.1, 0.9]), f, u, v, w, n, 1e-3, 0.5, rot)
+    assert abs(fw[0]) < 1e-16
+    # gas at rest at the wall temperature: the pressure p = rho / (2 lambda) alone
+    f = _max_2v([0.9, 0.0, 0.0, 0.9], u, v)
+    call("orc_flux_boundary_maxwell_1f2v", fw, ff, A([1.0, 0.0, 0.0, 0.9]), f, u, v, w, n, 1e-3, 0.5, rot)
+    assert np.allclose(fw, [0.0, 1e-3 * 0.5 * 0.5 * 0.9 / 0.9, 0.0, 0.0], atol=1e-13)
+
+
+def test_step_1f2v_bgk_fixed_point_relaxation_and_shakhov_refused():
+    """BGK (step_1f.jl:164-221): the cell's own Maxwellian is a fixed point; an anisotropic Gaussian's
+    directional temperature relaxes at the BGK rate; Shakhov is refused (step_1f.jl:194 reads an
+    undefined b)."""
+    u, v, w = tensor_grid(40, -8.0, 8.0, 2)
+    gam, mu, dt = 2.0, 0.02, 0.004
+    n = u.size
+    z4, zn = A(np.zeros(4)), A(np.zeros(n))
+
+    def step(f, W, P, coll=0):
+        f, W, P = A(f.copy()), A(W.copy()), A(P.copy())
+        RES, AVG = A(np.zeros(4)), A(np.zeros(4))
+        rc = call("orc_step_1f2v", W, P, f, z4, zn, z4, zn, z4, zn, z4, zn, u, v, w, n, gam, mu, 0.72, 1.0, 0.01, dt,
+                  RES, AVG, coll)
+        return rc, f, P
+
+    def moments(f):
+        return A([np.dot(w, f), np.dot(w, u * f), np.dot(w, v * f), 0.5 * np.dot(w, (u * u + v * v) * f)])
+
+    prim = [1.2, 0.3, -0.1, 0.7]
+    M = _max_2v(prim, u, v)
+    W = moments(M)
+    P = A(np.zeros(4)); call("orc_conserve_prim", W, 4, gam, P)
+    assert np.allclose(P, prim, rtol=1e-12)
+    rc, f, _ = step(M, W, P)
+    assert rc == 0 and np.allclose(f, M, rtol=1e-12)
+    # Tx != Ty with the same mean temperature
+    lx, ly = 0.5, 1.0 / (2.0 / prim[3] - 1.0 / 0.5)
+    f0 = A(prim[0] * np.sqrt(lx / np.pi) * np.sqrt(ly / np.pi) * np.exp(-lx * (u - prim[1]) ** 2 - ly * (v - prim[2]) ** 2))
+    W0 = moments(f0)
+    P0 = A(np.zeros(4)); call("orc_conserve_prim", W0, 4, gam, P0)
+    assert P0[3] == pytest.approx(prim[3], rel=1e-10)
+    rc, f, _ = step(f0, W0, P0)
+    x = dt / (mu * 2 * P0[3] ** (1 - 0.72) / P0[0])
+    cx2 = lambda g: np.dot(w, (u - P0[1]) ** 2 * g) / np.dot(w, g)
+    assert cx2(f) - 1 / (2 * P0[3]) == pytest.approx((cx2(f0) - 1 / (2 * P0[3])) / (1 + x), rel=1e-9)
+    assert np.allclose(moments(f), W0, rtol=1e-11, atol=1e-13)
+    rc, *_ = step(M, W, P, coll=1)
+    assert rc != 0
+
+
+def test_cavity_1f2v_symmetry_and_mass():
+    """The 2D1F2V driver (one distribution): walls at rest and a bump symmetric under x <-> y keep the
+    transpose symmetry and the total mass over 10 steps, KFVS and KCU."""
+    n = 10
+    for flux in ("kfvs", "kcu"):
+        cfg = kb.configs.cavity_2d1f2v(nx=n, ny=n, nu=12, nv=12, uLid=0.0, knudsen=0.1, flux=flux)
+        ks = kb.SolverSet(cfg)
+        ctr, face = kb.init_fvm(ks)
+        X, Y = ks.ps.x, ks.ps.y
+        bump = 0.2 * np.exp(-((X - 0.5) ** 2 + (Y - 0.3) ** 2) / 0.02) + 0.2 * np.exp(-((X - 0.3) ** 2 + (Y - 0.5) ** 2) / 0.02)
+        for j in range(n):
+            for i in range(n):
+                prim = np.array([1.0 + bump[j, i], 0.0, 0.0, 1.0])
+                ctr.w[j, i] = kb.theory.prim_conserve(prim, ks.gas.gamma)
+                ctr.prim[j, i] = prim
+                ctr.f[j, i, 0] = kb.theory.maxwellian_2v(ks.vs.u, ks.vs.v, prim[0], 0.0, 0.0, prim[3]).ravel()
+        m0 = np.sum(ctr.w[..., 0] * ks.ps.dx * ks.ps.dy)
+        dt = kb.timestep(ks, ctr)
+        oracle.OracleSolve(ks, ctr, face).run(dt, 10)
+        f = ctr.f.reshape(n, n, 12, 12)  # [j, i, iv, iu]
+        assert np.max(np.abs(f - np.transpose(f, (1, 0, 3, 2)))) <= 1e-11 * np.max(np.abs(f))
+        assert np.sum(ctr.w[..., 0] * ks.ps.dx * ks.ps.dy) == pytest.approx(m0, rel=1e-12)
+        assert ks.gas.gamma == 2.0

@@ -73,6 +73,23 @@ def test_chunks_never_overshoot_and_all_finish():
     assert st.summary()["requests"] == 3 and not be.live
 
 
+def test_requests_advance_in_pairs_from_their_own_start():
+    """Every request sees the step sequence of a solo run: pairs from its first step, then at most one
+    single step at its end (a two-step pass may round differently from two one-step passes)."""
+    reqs = [R(0, 25, 0.0), R(1, 7, 0.0), R(2, 40, 0.005), R(3, 13, 0.011)]
+    srv, be, clock = make(reqs, budget=100, chunk=7)
+    srv.run(reqs)
+    for r in reqs:
+        seq = [n for h, n in be.calls if r.rid in h]
+        assert sum(seq) == r.nsteps
+        assert all(n % 2 == 0 for n in seq[:-1]), (r.rid, seq)
+        assert seq[-1] % 2 == 0 or seq[-1] == 1, (r.rid, seq)
+        # a single step is the request's own last one, taken with no request that still continues
+        for h, n in be.calls:
+            if n == 1 and r.rid in h:
+                assert all(seq2[-1] == 1 for seq2 in ([m for hh, m in be.calls if q in hh] for q in h))
+
+
 def test_admission_by_memory_is_fifo():
     # budget fits two requests; the third waits for the first to finish
     reqs = [R(0, 10, 0.0, 40), R(1, 30, 0.0, 40), R(2, 10, 0.0, 40)]
