@@ -163,6 +163,21 @@ int kbg_step(kbg_solve *s, double dt, int nsteps, double *residual);
 int kbg_batch_step(kbg_ctx *ctx, kbg_solve *const *solves, int n, const double *dts, int nsteps,
                    double *residuals);
 
+/* kbg_batch_step without any device allocation.  No counterpart in KitBase: it is for callers that
+ * cannot allocate device memory while they step (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, kernels and results as kbg_batch_step; the
+ * batch's device staging (1D descriptors, work items, packs and residual partial rows; 2D
+ * descriptors and prefix sums, about 0.3 KB per solve and 0.13 KB per 56-60 cells of a 1D solve)
+ * lives in a fixed per-device array of KBG_NOALLOC_SCRATCH_BYTES that the library reserves for the
+ * life of the process.  A batch that needs more staging fails with KBG_ERR_ARG, as does one with a
+ * solve stepped by the phase kernels (1D1F3V, very large velocity grids) whose slope and face
+ * buffers no earlier call has allocated (KBG_ERR_STATE).  HIP graph replay is not used.  The array
+ * is shared by every context on a device and keeps the last call's contents: calls that overlap in
+ * time on one GPU (from two host threads) race; serialise them. */
+#define KBG_NOALLOC_SCRATCH_BYTES ((size_t)8 << 20)
+int kbg_batch_step_noalloc(kbg_ctx *ctx, kbg_solve *const *solves, int n, const double *dts, int nsteps,
+                           double *residuals);
+
 #ifdef __cplusplus
 }
 #endif

@@ -77,6 +77,7 @@ _PROTOS = {
     "kbg_step": (C.c_int, [C.c_void_p, C.c_double, C.c_int, _dp]),
     "kbg_batch_step": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_int, _dp, C.c_int, _dp]),
     "kbg_solve_health": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_int]),
+    "kbg_batch_step_noalloc": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_int, _dp, C.c_int, _dp]),
 }
 
 EXPORTED = tuple(_PROTOS)

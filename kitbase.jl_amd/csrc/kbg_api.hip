@@ -10,6 +10,7 @@
 #include <cstring>
 #include <initializer_list>
 #include <map>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -515,7 +516,39 @@ void build_items(kbg_solve* s, int seg, int solve_index, int nw, std::vector<Wor
 
 int step_phase_1d(kbg_solve* s, double dt, double* residual, hipStream_t st);
 int step_phase_2d(kbg_solve* s, double dt, double* residual, hipStream_t st);
-int batch_step_impl(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double* dts, int nsteps, double* residuals);
+int batch_step_impl(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double* dts, int nsteps, double* residuals, bool noalloc);
+
+// Staging of kbg_batch_step_noalloc: a fixed device array per GPU, held until the process ends.
+// Host code never passes `scratch` itself to a kernel or a copy (in host code the name is a host-side
+// stand-in, not a device address): kbg_ctx_create looks up each device's address once, with that
+// device current, and every use goes through scratch_ptr(device).
+constexpr size_t kNoallocScratchBytes = KBG_NOALLOC_SCRATCH_BYTES;
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_addr[kMaxDevices] = {};
+
+void* scratch_ptr(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return device >= 0 && device < kMaxDevices ? g_scratch_addr[device] : nullptr;
+}
+
+}  // namespace
+
+static __device__ __align__(256) char scratch[kNoallocScratchBytes];
+
+namespace {
+
+// called by kbg_ctx_create with `device` current
+int setup_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices) return fail(KBG_ERR_ARG, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_addr[device]) return KBG_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return fail(KBG_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_addr[device] = p;
+    return KBG_OK;
+}
 
 }  // namespace
 
@@ -532,6 +565,7 @@ int kbg_ctx_create(int device, kbg_ctx** out) {
     if (e != hipSuccess || n <= 0) return fail(KBG_ERR_HIP, "no HIP device available");
     if (device < 0 || device >= n) return fail(KBG_ERR_ARG, "device index out of range");
     HIPCHK(hipSetDevice(device));
+    if (int rc = setup_scratch(device)) return rc;
     kbg_ctx* c = new kbg_ctx();
     c->device = device;
     e = hipStreamCreate(&c->stream);
@@ -1046,17 +1080,22 @@ int kbg_step(kbg_solve* s, double dt, int nsteps, double* residual) {
 }
 
 int kbg_batch_step(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double* dts, int nsteps, double* residuals) {
-    return batch_step_impl(ctx, solves, n, dts, nsteps, residuals);
+    return batch_step_impl(ctx, solves, n, dts, nsteps, residuals, false);
+}
+
+int kbg_batch_step_noalloc(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double* dts, int nsteps, double* residuals) {
+    return batch_step_impl(ctx, solves, n, dts, nsteps, residuals, true);
 }
 
 }  // extern "C"
 
 namespace {
 
-// kbg_batch_step.  The batch's device staging (1D descriptors, work items, packs, residual partial
-// rows; 2D descriptors and prefix sums) lives in the context's buffers, grown with hipMalloc when a
-// batch outgrows them.
-int batch_step_impl(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double* dts, int nsteps, double* residuals) {
+// kbg_batch_step and its allocation-free variant.  The batch's device staging (1D descriptors, work
+// items, packs, residual partial rows; 2D descriptors and prefix sums) lives in the context's
+// buffers, grown with hipMalloc when a batch outgrows them, or -- noalloc -- in this device's
+// file-scope `scratch` array, carved per call.
+int batch_step_impl(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double* dts, int nsteps, double* residuals, bool noalloc) {
     if (!ctx || (n > 0 && (!solves || !dts))) return fail(KBG_ERR_ARG, "NULL argument");
     if (nsteps <= 0 || n <= 0) return KBG_OK;
     hipStream_t st = ctx->stream;
@@ -1155,6 +1194,9 @@ int batch_step_impl(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double*
     // solves stepped by the phase kernels: buffers and descriptors before the streams fork
     for (int i : others) {
         kbg_solve* s = solves[i];
+        if (noalloc && !(s->d.dim == 1 ? s->h1.sw != nullptr : s->h2.sf != nullptr))
+            return fail(KBG_ERR_STATE, "kbg_batch_step_noalloc: a solve stepped by the phase kernels has no slope and face "
+                                       "buffers yet (one kbg_step, kbg_batch_step or phase-level call allocates them)");
         if ((rc = ensure_phase_buffers(s))) return rc;
         if (s->d.dim == 1) {
             s->h1.dt = dts[i];
@@ -1222,52 +1264,73 @@ int batch_step_impl(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double*
         L.npacks = packs.size() - L.pack0;
         launches.push_back(std::move(L));
     }
-    // device staging of the batch: the context's buffers (grown on demand)
+    // device staging of the batch: the context's buffers (grown on demand) or this device's scratch
     Solve1D* d_descs = nullptr;
     WorkItem* d_items = nullptr;
     double* d_rpart = nullptr;
     kbg::Pack* d_packs = nullptr;
     Solve2D* d_descs2 = nullptr;
     int* d_pref2 = nullptr;
-    if (descs.size() > ctx->descs_cap) {
-        (void)hipFree(ctx->d_descs);
-        ctx->d_descs = nullptr;
-        ctx->descs_cap = descs.size() * 2;
-        HIPCHK(hipMalloc((void**)&ctx->d_descs, ctx->descs_cap * sizeof(Solve1D)));
+    if (noalloc) {
+        char* base = static_cast<char*>(scratch_ptr(ctx->device));
+        if (!base) return fail(KBG_ERR_STATE, "kbg_batch_step_noalloc: the device's scratch address was not set up");
+        size_t off = 0;
+        auto carve = [&](size_t bytes) {
+            char* p = base + off;
+            off += (bytes + 255) / 256 * 256;
+            return p;
+        };
+        d_descs = reinterpret_cast<Solve1D*>(carve(descs.size() * sizeof(Solve1D)));
+        d_items = reinterpret_cast<WorkItem*>(carve(items.size() * sizeof(WorkItem)));
+        d_rpart = reinterpret_cast<double*>(carve(items.size() * 12 * sizeof(double)));
+        d_packs = reinterpret_cast<kbg::Pack*>(carve(packs.size() * sizeof(kbg::Pack)));
+        d_descs2 = reinterpret_cast<Solve2D*>(carve(descs2.size() * sizeof(Solve2D)));
+        d_pref2 = reinterpret_cast<int*>(carve(pref2.size() * sizeof(int)));
+        if (off > kNoallocScratchBytes)
+            return fail(KBG_ERR_ARG, "kbg_batch_step_noalloc: the batch needs " + std::to_string(off) +
+                                         " bytes of staging; the allocation-free entry point holds at most " +
+                                         std::to_string(kNoallocScratchBytes) + " (use kbg_batch_step or a smaller batch)");
+    } else {
+        if (descs.size() > ctx->descs_cap) {
+            (void)hipFree(ctx->d_descs);
+            ctx->d_descs = nullptr;
+            ctx->descs_cap = descs.size() * 2;
+            HIPCHK(hipMalloc((void**)&ctx->d_descs, ctx->descs_cap * sizeof(Solve1D)));
+        }
+        if (items.size() > ctx->items_cap) {
+            (void)hipFree(ctx->d_items);
+            (void)hipFree(ctx->d_rpart);
+            ctx->d_items = nullptr;
+            ctx->d_rpart = nullptr;
+            ctx->items_cap = items.size() * 2;
+            HIPCHK(hipMalloc((void**)&ctx->d_items, ctx->items_cap * sizeof(WorkItem)));
+            HIPCHK(hipMalloc((void**)&ctx->d_rpart, ctx->items_cap * 12 * sizeof(double)));
+        }
+        if (packs.size() > ctx->packs_cap) {
+            (void)hipFree(ctx->d_packs);
+            ctx->d_packs = nullptr;
+            ctx->packs_cap = packs.size() * 2;
+            HIPCHK(hipMalloc((void**)&ctx->d_packs, ctx->packs_cap * sizeof(kbg::Pack)));
+        }
+        if (descs2.size() > ctx->descs2_cap) {
+            (void)hipFree(ctx->d_descs2);
+            ctx->d_descs2 = nullptr;
+            ctx->descs2_cap = descs2.size() * 2;
+            HIPCHK(hipMalloc((void**)&ctx->d_descs2, ctx->descs2_cap * sizeof(Solve2D)));
+        }
+        if (pref2.size() > ctx->pref2_cap) {
+            (void)hipFree(ctx->d_pref2);
+            ctx->d_pref2 = nullptr;
+            ctx->pref2_cap = pref2.size() * 2;
+            HIPCHK(hipMalloc((void**)&ctx->d_pref2, ctx->pref2_cap * sizeof(int)));
+        }
+        d_descs = ctx->d_descs;
+        d_items = ctx->d_items;
+        d_rpart = ctx->d_rpart;
+        d_packs = ctx->d_packs;
+        d_descs2 = ctx->d_descs2;
+        d_pref2 = ctx->d_pref2;
     }
-    if (items.size() > ctx->items_cap) {
-        (void)hipFree(ctx->d_items);
-        (void)hipFree(ctx->d_rpart);
-        ctx->d_items = nullptr;
-        ctx->d_rpart = nullptr;
-        ctx->items_cap = items.size() * 2;
-        HIPCHK(hipMalloc((void**)&ctx->d_items, ctx->items_cap * sizeof(WorkItem)));
-        HIPCHK(hipMalloc((void**)&ctx->d_rpart, ctx->items_cap * 12 * sizeof(double)));
-    }
-    if (packs.size() > ctx->packs_cap) {
-        (void)hipFree(ctx->d_packs);
-        ctx->d_packs = nullptr;
-        ctx->packs_cap = packs.size() * 2;
-        HIPCHK(hipMalloc((void**)&ctx->d_packs, ctx->packs_cap * sizeof(kbg::Pack)));
-    }
-    if (descs2.size() > ctx->descs2_cap) {
-        (void)hipFree(ctx->d_descs2);
-        ctx->d_descs2 = nullptr;
-        ctx->descs2_cap = descs2.size() * 2;
-        HIPCHK(hipMalloc((void**)&ctx->d_descs2, ctx->descs2_cap * sizeof(Solve2D)));
-    }
-    if (pref2.size() > ctx->pref2_cap) {
-        (void)hipFree(ctx->d_pref2);
-        ctx->d_pref2 = nullptr;
-        ctx->pref2_cap = pref2.size() * 2;
-        HIPCHK(hipMalloc((void**)&ctx->d_pref2, ctx->pref2_cap * sizeof(int)));
-    }
-    d_descs = ctx->d_descs;
-    d_items = ctx->d_items;
-    d_rpart = ctx->d_rpart;
-    d_packs = ctx->d_packs;
-    d_descs2 = ctx->d_descs2;
-    d_pref2 = ctx->d_pref2;
     if (!descs.empty()) {
         HIPCHK(hipMemcpyAsync(d_descs, descs.data(), descs.size() * sizeof(Solve1D), hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(d_items, items.data(), items.size() * sizeof(WorkItem), hipMemcpyHostToDevice, st));
@@ -1357,7 +1420,7 @@ int batch_step_impl(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double*
     // at both parities of the two-step launches, captured once per launch plan (the signature below: every launch's shape, stream and
     // arguments; descriptors are read from device memory, re-uploaded by every call) and reused
     // by later calls on the same batch.
-    const bool use_graph = ctx->graphs && !residuals && nsteps >= 2;
+    const bool use_graph = ctx->graphs && !residuals && nsteps >= 2 && !noalloc;
     std::string sig;
     if (use_graph) {
         auto put = [&](const void* p, size_t b) { sig.append(static_cast<const char*>(p), b); };
