@@ -2,8 +2,9 @@
  * kbgpu.h -- C ABI of libkbgpu.so, the MI355X (gfx950) discrete-velocity time stepper that replaces
  * KitBase.jl's CPU `reconstruct!` / `evolve!` / `update!` hot loop.
  *
- * Plain pointers and sizes only.  All host arrays are FP64, contiguous, owned by the caller, and
- * laid out the way KitBase stores the same data once its OffsetArrays are flattened:
+ * Plain pointers and sizes only.  All host arrays are FP64, contiguous, owned by the caller (they
+ * need to live only through the call that takes them: kbg_solve_create copies what a descriptor
+ * points to), and laid out the way KitBase stores the same data once its OffsetArrays are flattened:
  *   1D  cells  k = i - 1 + ng for the physical index i in [1-ng, nx+ng]  (N = nx + 2 ng rows)
  *       faces  k - 1 for the physical face k in [1, nx+1] (between cells k-1 and k)
  *   2D  cells  (i-1) + nx (j-1);  x-faces (i-1) + (nx+1)(j-1), i in [1, nx+1];

@@ -95,7 +95,11 @@ struct kbg_solve {
     size_t esz = 4;
     int nw = 3, N = 0, RS = 0;
     bool fused = false;   // stepped by the fused sweep (cell-fastest f)
-    bool rep_full = false;  // FP32 swept solve whose f buffer holds f, not its departure from M~(prim)
+    bool rep_full = false;  // FP32 solve stored as departures whose buffers hold full values for now
+    std::vector<float> mref;  // 2D departure solves: the reference Maxwellian (h, b) as stored
+    // copies of the host arrays the descriptor pointed to (the caller's arrays need only live through
+    // kbg_solve_create); d's pointers are redirected to them
+    std::vector<double> hu, hv, hw, hwt, hdx, hdy, hbc, hxgeo, hygeo, hxwall, hywall;
     int thr2 = 0;         // 2D: threads per workgroup of the fused step (0: phase kernels)
     int sweep_nw = 0;     // wavefronts per sweep workgroup
     double vtab_dt = -1;  // dt the velocity table was built for
@@ -252,10 +256,62 @@ int get_f1d_delta(kbg_solve* s, double* dst, int buf) {
 
 // the phase-level kernels work on f itself: convert a departure-form solve once (in place, FP32)
 int to_full_rep(kbg_solve* s) {
-    if (s->d.dim != 1 || !s->h1.delta || s->rep_full) return KBG_OK;
-    HIPCHK(kbg::launch_delta1d(s->h1, s->d1, nullptr, 2, s->h1.cur, s->ctx->stream));
-    s->rep_full = true;
+    if (s->rep_full) return KBG_OK;
+    if (s->d.dim == 1 && s->h1.delta) {
+        HIPCHK(kbg::launch_delta1d(s->h1, s->d1, nullptr, 2, s->h1.cur, s->ctx->stream));
+        s->rep_full = true;
+    } else if (s->d.dim == 2 && s->h2.delta) {
+        HIPCHK(kbg::launch_delta2d(s->h2, s->d2, 2, s->ctx->stream));
+        s->rep_full = true;
+    }
     return KBG_OK;
+}
+
+// The reference Maxwellian of a 2D departure solve: the Maxwellian of the mean conserved state over
+// the cells (host w) with rho, T and U / sqrt(T), V / sqrt(T) rounded to a grid of 1/16 (in log2 for
+// rho and T), so a state uploaded again later in the run (a checkpoint) usually gives the same
+// reference; b = h K / (2 lambda); rounded to FP32 as stored; its moments in FP64
+void set_mref2d(kbg_solve* s, const double* w) {
+    const kbg_desc& d = s->d;
+    Solve2D& S = s->h2;
+    const size_t C = s->N;
+    double wm[4] = {0, 0, 0, 0}, p[4];
+    for (size_t c = 0; c < C; c++)
+        for (int m = 0; m < 4; m++) wm[m] += w[c * 4 + m];
+    for (int m = 0; m < 4; m++) wm[m] /= double(C);
+    kbg::conserve_prim<4>(wm, d.gamma, p);
+    {
+        const double q = 16.0;
+        const double T = std::exp2(std::round(q * std::log2(0.5 / p[3])) / q), c = std::sqrt(T);
+        p[0] = std::exp2(std::round(q * std::log2(p[0])) / q);
+        p[1] = std::round(q * p[1] / c) / q * c;
+        p[2] = std::round(q * p[2] / c) / q * c;
+        p[3] = 0.5 / T;
+    }
+    const int nv = d.nv;
+    s->mref.assign(2 * (size_t)nv, 0.0f);
+    const double kb = S.K / (2.0 * p[3]);
+    for (int m = 0; m < 11; m++) S.mmom[m] = 0.0;
+    for (int v = 0; v < nv; v++) {
+        const double u = d.u[v], vv = d.v[v], wq = d.weights[v];
+        const double cu = u - p[1], cv = vv - p[2];
+        const float mh = float(p[0] * (p[3] / kbg::kPi) * std::exp(-p[3] * (cu * cu + cv * cv)));
+        const float mb = float(double(mh) * kb);
+        s->mref[v] = mh;
+        s->mref[nv + v] = mb;
+        const double a = wq * double(mh), b = wq * double(mb);
+        S.mmom[0] += a * u;
+        S.mmom[1] += a * vv;
+        S.mmom[2] += a * u * u;
+        S.mmom[3] += a * u * vv;
+        S.mmom[4] += a * vv * vv;
+        S.mmom[5] += a * u * u * u;
+        S.mmom[6] += a * u * u * vv;
+        S.mmom[7] += a * u * vv * vv;
+        S.mmom[8] += a * vv * vv * vv;
+        S.mmom[9] += b * u;
+        S.mmom[10] += b * vv;
+    }
 }
 
 // per-velocity constants of the sweep for time step dt, in chunks of 8 (kbg_sweep.h VChunk)
@@ -332,7 +388,12 @@ int validate(const kbg_desc* d) {
         if (d->nf == 2 && (d->bc[0] == KBG_BC_MAXWELL || d->bc[1] == KBG_BC_MAXWELL) && !d->bc_prim)
             return fail(KBG_ERR_ARG, "Maxwell walls need bc_prim");
     } else if (d->dim == 2) {
-        if (d->nf != 2 || d->vdim != 2) return fail(KBG_ERR_UNSUPPORTED, "2D supports 2F2V");
+        if ((d->nf != 1 && d->nf != 2) || d->vdim != 2) return fail(KBG_ERR_UNSUPPORTED, "2D supports 2F2V and 1F2V");
+        if (d->nf == 1 && d->flux == KBG_UGKS) return fail(KBG_ERR_UNSUPPORTED, "KitBase has no 2D1F2V UGKS flux (flux_ugks.jl covers 2D2F2V)");
+        if (d->nf == 1 && d->collision == KBG_SHAKHOV)
+            return fail(KBG_ERR_UNSUPPORTED, "2D1F2V Shakhov step! references an undefined b (step_1f.jl:194); only BGK is defined");
+        if (d->nf == 1 && d->K != 0.0)
+            return fail(KBG_ERR_UNSUPPORTED, "2D1F2V is built for K = 0 (no internal degrees of freedom)");
         if (d->nx < 3 || d->ny < 3) return fail(KBG_ERR_ARG, "2D needs nx, ny >= 3");
         if (d->ng != 0) return fail(KBG_ERR_UNSUPPORTED, "2D ghost layers are not supported");
         if (!d->v || !d->dy || !d->xgeo || !d->ygeo) return fail(KBG_ERR_ARG, "2D needs v, dy, xgeo, ygeo");
@@ -419,6 +480,9 @@ void fill2d(kbg_solve* s) {
     S.limiter = d.limiter;
     S.interp2 = d.interp_order == 2;
     for (int k = 0; k < 4; k++) S.bc[k] = d.bc[k];
+    // 2D1F2V (K = 0) runs in the two-distribution layout with b = 0: with no internal degrees of
+    // freedom every b term vanishes, which leaves KitBase's 1F2V step (step_1f.jl:164-221), KFVS / KCU
+    // flux (flux_kfvs.jl:615-647, flux_kcu.jl:828-891) and wall (flux_boundary.jl:160-183)
     S.K = d.K;
     S.gam = d.gamma;
     S.mu = d.mu_ref;
@@ -690,6 +754,7 @@ size_t kbg_solve_bytes(const kbg_desc* d) {
     size_t b = C * 2 * nv * esz + F * 2 * nv * esz + C * (2 * 4 * 8 + 16 * 8 + 2 * 8) + F * (3 + 4 + 4) * 8 + 3 * nv * esz;
     if (d->flux != KBG_KFVS) b += F * kbg::kFaceCoef * 8;
     if (kbg::step2d_threads(d->precision == KBG_FP64, d->nv) == 0) b += C * 8 * 8 + C * 4 * nv * esz;  // slopes
+    else if (esz == 4) b += 2 * nv * 4 + F * 8;  // departure form: reference Maxwellian, per-face coefficients
     return b;
 }
 
@@ -701,6 +766,33 @@ int kbg_solve_create(kbg_ctx* ctx, const kbg_desc* desc, kbg_solve** out) {
     kbg_solve* s = new kbg_solve();
     s->ctx = ctx;
     s->d = *desc;
+    {
+        kbg_desc& dd = s->d;
+        auto keep = [](std::vector<double>& v, const double*& ptr, size_t n) {
+            if (!ptr) return;
+            v.assign(ptr, ptr + n);
+            ptr = v.data();
+        };
+        const size_t nv = dd.nv;
+        keep(s->hu, dd.u, nv);
+        keep(s->hv, dd.v, nv);
+        keep(s->hw, dd.w, nv);
+        keep(s->hwt, dd.weights, nv);
+        if (dd.dim == 1) {
+            keep(s->hdx, dd.dx, size_t(dd.nx) + 2 * size_t(dd.ng));
+            keep(s->hbc, dd.bc_prim, 2 * size_t(nw_of(desc)));
+            dd.dy = dd.xgeo = dd.ygeo = dd.xwall = dd.ywall = nullptr;
+        } else {
+            const size_t C = size_t(dd.nx) * dd.ny, FX = size_t(dd.nx + 1) * dd.ny, FY = size_t(dd.nx) * (dd.ny + 1);
+            keep(s->hdx, dd.dx, C);
+            keep(s->hdy, dd.dy, C);
+            keep(s->hxgeo, dd.xgeo, FX * 3);
+            keep(s->hygeo, dd.ygeo, FY * 3);
+            keep(s->hxwall, dd.xwall, FX * 4);
+            keep(s->hywall, dd.ywall, FY * 4);
+            dd.bc_prim = nullptr;
+        }
+    }
     s->fp64 = desc->precision == KBG_FP64;
     s->esz = s->fp64 ? 8 : 4;
     s->nw = nw_of(desc);
@@ -837,6 +929,20 @@ int kbg_solve_create(kbg_ctx* ctx, const kbg_desc* desc, kbg_solve** out) {
         S.rcell = s->rcell;
         // the fused step addresses f and the face fluxes with 32-bit byte offsets
         s->thr2 = FX * 2 * d.nv * s->esz < (1ull << 31) && FY * 2 * d.nv * s->esz < (1ull << 31) ? kbg::step2d_threads(s->fp64, d.nv) : 0;
+        // FP32 solves of the fused step store departures from a reference Maxwellian (set by upload)
+        S.delta = !s->fp64 && s->thr2 > 0;
+        if (S.delta) {
+            if ((rc = dev_alloc(s, &p, 2 * size_t(d.nv) * 4))) return bail(rc);
+            HIPCHK(hipMemset(p, 0, 2 * size_t(d.nv) * 4));
+            S.mref = (const float*)p;
+            s->mref.assign(2 * size_t(d.nv), 0.0f);
+            if ((rc = dev_alloc(s, &p, FX * 8))) return bail(rc);
+            HIPCHK(hipMemset(p, 0, FX * 8));
+            S.xcref = (double*)p;
+            if ((rc = dev_alloc(s, &p, FY * 8))) return bail(rc);
+            HIPCHK(hipMemset(p, 0, FY * 8));
+            S.ycref = (double*)p;
+        }
         if ((rc = dev_alloc(s, &p, 8 * 8))) return bail(rc);
         s->res = (double*)p;
         if ((rc = dev_alloc(s, &p, 8))) return bail(rc);
@@ -873,6 +979,18 @@ int kbg_solve_health(kbg_solve* s, int* flags, int clear) {
     return KBG_OK;
 }
 
+// 2D1F2V host arrays [rows][1][nv] <-> the two-distribution device layout [rows][2][nv] (b = 0)
+std::vector<double> widen2d(const kbg_solve* s, const double* src, size_t rows) {
+    const size_t nv = s->d.nv;
+    std::vector<double> out(rows * 2 * nv, 0.0);
+    for (size_t r = 0; r < rows; r++) std::copy(src + r * nv, src + (r + 1) * nv, out.begin() + r * 2 * nv);
+    return out;
+}
+void narrow2d(const kbg_solve* s, const std::vector<double>& src, double* dst, size_t rows) {
+    const size_t nv = s->d.nv;
+    for (size_t r = 0; r < rows; r++) std::copy(src.begin() + r * 2 * nv, src.begin() + r * 2 * nv + nv, dst + r * nv);
+}
+
 int kbg_upload(kbg_solve* s, const double* w, const double* prim, const double* f) {
     if (!s || !w || !prim || !f) return fail(KBG_ERR_ARG, "NULL argument");
     int rc;
@@ -893,7 +1011,25 @@ int kbg_upload(kbg_solve* s, const double* w, const double* prim, const double* 
     Solve2D& S = s->h2;
     if ((rc = put_f64(s, S.w, w, N * 4))) return rc;
     if ((rc = put_f64(s, S.prim, prim, N * 4))) return rc;
-    return put_vec(s, S.f, f, N * 2 * s->d.nv);
+    std::vector<double> wide;
+    if (s->d.nf == 1) {
+        wide = widen2d(s, f, N);
+        f = wide.data();
+    }
+    if (!S.delta) return put_vec(s, S.f, f, N * 2 * s->d.nv);
+    // departures from a reference Maxwellian of this state; the face fluxes count as full values
+    set_mref2d(s, w);
+    const size_t RS = 2 * (size_t)s->d.nv, FX = size_t(s->d.nx + 1) * s->d.ny, FY = size_t(s->d.nx) * (s->d.ny + 1);
+    std::vector<float> tmp(N * RS);
+    for (size_t c = 0; c < N; c++)
+        for (size_t k = 0; k < RS; k++) tmp[c * RS + k] = float(f[c * RS + k] - double(s->mref[k]));
+    HIPCHK(hipMemcpyAsync((void*)S.mref, s->mref.data(), RS * 4, hipMemcpyHostToDevice, s->ctx->stream));
+    HIPCHK(hipMemcpyAsync(S.f, tmp.data(), N * RS * 4, hipMemcpyHostToDevice, s->ctx->stream));
+    HIPCHK(hipMemsetAsync(S.xcref, 0, FX * 8, s->ctx->stream));
+    HIPCHK(hipMemsetAsync(S.ycref, 0, FY * 8, s->ctx->stream));
+    HIPCHK(hipStreamSynchronize(s->ctx->stream));
+    s->rep_full = false;
+    return push2d(s);
 }
 
 int kbg_download(kbg_solve* s, double* w, double* prim, double* f) {
@@ -916,7 +1052,21 @@ int kbg_download(kbg_solve* s, double* w, double* prim, double* f) {
     Solve2D& S = s->h2;
     if (w && (rc = get_f64(s, w, S.w, N * 4))) return rc;
     if (prim && (rc = get_f64(s, prim, S.prim, N * 4))) return rc;
-    if (f && (rc = get_vec(s, f, S.f, N * 2 * s->d.nv))) return rc;
+    if (!f) return KBG_OK;
+    std::vector<double> wide;
+    double* fo = f;
+    if (s->d.nf == 1) {
+        wide.resize(N * 2 * s->d.nv);
+        fo = wide.data();
+    }
+    if ((rc = get_vec(s, fo, S.f, N * 2 * s->d.nv))) return rc;
+    if (S.delta && !s->rep_full) {
+        // departure + Mr, exactly (two FP32 values summed in FP64)
+        const size_t RS = 2 * (size_t)s->d.nv;
+        for (size_t c = 0; c < N; c++)
+            for (size_t k = 0; k < RS; k++) fo[c * RS + k] += double(s->mref[k]);
+    }
+    if (s->d.nf == 1) narrow2d(s, wide, f, N);
     return KBG_OK;
 }
 
@@ -932,6 +1082,7 @@ int kbg_upload_slopes(kbg_solve* s, const double* sw, const double* sf) {
     }
     if ((rc = ensure_phase_buffers(s))) return rc;
     if (sw && (rc = put_f64(s, s->h2.sw, sw, N * 8))) return rc;
+    if (sf && s->d.nf == 1) return put_vec(s, s->h2.sf, widen2d(s, sf, 2 * N).data(), N * 4 * s->d.nv);
     if (sf && (rc = put_vec(s, s->h2.sf, sf, N * 4 * s->d.nv))) return rc;
     return KBG_OK;
 }
@@ -948,6 +1099,12 @@ int kbg_download_slopes(kbg_solve* s, double* sw, double* sf) {
     }
     if ((rc = ensure_phase_buffers(s))) return rc;
     if (sw && (rc = get_f64(s, sw, s->h2.sw, N * 8))) return rc;
+    if (sf && s->d.nf == 1) {
+        std::vector<double> wide(N * 4 * s->d.nv);
+        if ((rc = get_vec(s, wide.data(), s->h2.sf, wide.size()))) return rc;
+        narrow2d(s, wide, sf, 2 * N);
+        return KBG_OK;
+    }
     if (sf && (rc = get_vec(s, sf, s->h2.sf, N * 4 * s->d.nv))) return rc;
     return KBG_OK;
 }
@@ -964,11 +1121,21 @@ int kbg_upload_faces(kbg_solve* s, const double* fw, const double* ff) {
     }
     if ((rc = ensure_phase_buffers(s))) return rc;
     const size_t FX = size_t(s->d.nx + 1) * s->d.ny, FY = size_t(s->d.nx) * (s->d.ny + 1), RS = 2 * s->d.nv;
+    if (ff && s->h2.delta && !s->rep_full) {
+        // uploaded face fluxes are full values: none leaves out the reference Maxwellian's part
+        HIPCHK(hipMemsetAsync(s->h2.xcref, 0, FX * 8, s->ctx->stream));
+        HIPCHK(hipMemsetAsync(s->h2.ycref, 0, FY * 8, s->ctx->stream));
+    }
     if (fw) {
         if ((rc = put_f64(s, s->h2.xfw, fw, FX * 4))) return rc;
         if ((rc = put_f64(s, s->h2.yfw, fw + FX * 4, FY * 4))) return rc;
     }
     if (ff) {
+        std::vector<double> wide;
+        if (s->d.nf == 1) {
+            wide = widen2d(s, ff, FX + FY);
+            ff = wide.data();
+        }
         if ((rc = put_vec(s, s->h2.xff, ff, FX * RS))) return rc;
         if ((rc = put_vec(s, s->h2.yff, ff + FX * RS, FY * RS))) return rc;
     }
@@ -992,8 +1159,29 @@ int kbg_download_faces(kbg_solve* s, double* fw, double* ff) {
         if ((rc = get_f64(s, fw + FX * 4, s->h2.yfw, FY * 4))) return rc;
     }
     if (ff) {
-        if ((rc = get_vec(s, ff, s->h2.xff, FX * RS))) return rc;
-        if ((rc = get_vec(s, ff + FX * RS, s->h2.yff, FY * RS))) return rc;
+        std::vector<double> wide;
+        double* fo = ff;
+        if (s->d.nf == 1) {
+            wide.resize((FX + FY) * RS);
+            fo = wide.data();
+        }
+        if ((rc = get_vec(s, fo, s->h2.xff, FX * RS))) return rc;
+        if ((rc = get_vec(s, fo + FX * RS, s->h2.yff, FY * RS))) return rc;
+        if (s->h2.delta && !s->rep_full) {
+            // the fused step's face fluxes leave out c x Mr: add it back (FP64, on the host)
+            std::vector<double> c(FX + FY);
+            if ((rc = get_f64(s, c.data(), s->h2.xcref, FX)) || (rc = get_f64(s, c.data() + FX, s->h2.ycref, FY))) return rc;
+            const kbg_desc& d = s->d;
+            for (size_t fc = 0; fc < FX + FY; fc++) {
+                if (c[fc] == 0.0) continue;
+                const double* g = fc < FX ? d.xgeo + fc * 3 : d.ygeo + (fc - FX) * 3;
+                for (size_t k = 0; k < RS; k++) {
+                    const size_t iv = k % d.nv;
+                    fo[fc * RS + k] += c[fc] * (g[0] * d.u[iv] + g[1] * d.v[iv]) * double(s->mref[k]);
+                }
+            }
+        }
+        if (s->d.nf == 1) narrow2d(s, wide, ff, FX + FY);
     }
     return KBG_OK;
 }
@@ -1005,7 +1193,7 @@ int kbg_reconstruct(kbg_solve* s) {
         if ((rc = ensure_phase_buffers(s)) || (rc = push1d(s)) || (rc = to_full_rep(s))) return rc;
         HIPCHK(kbg::launch_recon1d(s->fp64, s->h1, s->d1, s->ctx->stream));
     } else {
-        if ((rc = ensure_phase_buffers(s)) || (rc = push2d(s))) return rc;
+        if ((rc = ensure_phase_buffers(s)) || (rc = push2d(s)) || (rc = to_full_rep(s))) return rc;
         HIPCHK(kbg::launch_recon2d(s->fp64, s->h2, s->d2, s->ctx->stream));
     }
     HIPCHK(hipStreamSynchronize(s->ctx->stream));
@@ -1025,7 +1213,7 @@ int kbg_evolve(kbg_solve* s, double dt) {
     } else {
         if ((rc = ensure_phase_buffers(s))) return rc;
         s->h2.dt = dt;
-        if ((rc = push2d(s))) return rc;
+        if ((rc = push2d(s)) || (rc = to_full_rep(s))) return rc;
         HIPCHK(kbg::launch_evolve2d(s->fp64, s->h2, s->d2, s->ctx->stream));
     }
     HIPCHK(hipStreamSynchronize(s->ctx->stream));
@@ -1043,7 +1231,7 @@ int kbg_update(kbg_solve* s, double dt, double* residual) {
     } else {
         if ((rc = ensure_phase_buffers(s))) return rc;
         s->h2.dt = dt;
-        if ((rc = push2d(s))) return rc;
+        if ((rc = push2d(s)) || (rc = to_full_rep(s))) return rc;
         rc = step_phase_2d(s, dt, residual, s->ctx->stream);
     }
     if (rc) return rc;
@@ -1346,6 +1534,14 @@ int batch_step_impl(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double*
         kbg_solve* s = solves[i];
         if (s->h1.delta && s->rep_full) {
             HIPCHK(kbg::launch_delta1d(s->h1, d_descs + desc_of[i], nullptr, 3, s->h1.cur, st));
+            s->rep_full = false;
+        }
+    }
+    for (int i : fused2) {
+        kbg_solve* s = solves[i];
+        if (s->h2.delta && s->rep_full) {
+            if ((rc = push2d(s))) return rc;
+            HIPCHK(kbg::launch_delta2d(s->h2, s->d2, 3, st));
             s->rep_full = false;
         }
     }

@@ -26,6 +26,8 @@
 // (reconstruct2! reconstruction.jl:48-50 on boundary rows); walls evolve_boundary.jl:52-178 ->
 // flux_boundary.jl:190-220; update! solver_update.jl:193-246 + update_boundary.jl:172-306;
 // step! step_2f.jl:179-249.
+#include <algorithm>
+
 #include "kbg_device.h"
 #include "kbg_types.h"
 
@@ -405,6 +407,40 @@ __global__ __launch_bounds__(flux2d_max_threads<T>()) void k_flux2d(const Solve2
                 for (int w = 1; w < NWV; w++) q += red[w * 16 + m];
                 r[m] = q;
             }
+            if (sizeof(T) == 4 && S.delta) {
+                // the sums ran over departures: add the reference Maxwellian's sums in FP64, from its
+                // moments in the global frame (x = n1 u + n2 v, y = n1 v - n2 u), and record its
+                // coefficient in this face's flux for the cell update
+                const double* mm = S.mmom;
+                const double S1 = n1 * mm[0] + n2 * mm[1];
+                const double Sx = n1 * n1 * mm[2] + 2.0 * n1 * n2 * mm[3] + n2 * n2 * mm[4];
+                const double Sy = (n1 * n1 - n2 * n2) * mm[3] + n1 * n2 * (mm[4] - mm[2]);
+                const double Sq = n1 * (mm[5] + mm[7]) + n2 * (mm[6] + mm[8]);
+                const double Sb = n1 * mm[9] + n2 * mm[10];
+                double cf;
+                if constexpr (FLUX == KFVS) {
+                    cf = dt * len;
+                    r[0] += cf * S1;
+                    r[1] += cf * Sx;
+                    r[2] += cf * Sy;
+                    r[3] += cf * Sq;
+                    r[4] += cf * Sb;
+                } else if constexpr (FLUX == KCU) {
+                    cf = (*(const KcuFace<4>*)rec).Mt2 * len;
+                    r[0] += S1;
+                    r[1] += Sx;
+                    r[2] += Sy;
+                    r[3] += Sq;
+                    r[4] += Sb;
+                } else {
+                    cf = (*(const UgksFace2*)rec).Mt[3] * len;
+                    r[0] += S1;
+                    r[2] += Sx;
+                    r[4] += Sy;
+                    r[6] += Sq + Sb;
+                }
+                (dir == 0 ? S.xcref : S.ycref)[fi] = cf;
+            }
             double l0, l1, l2, l3;
             if constexpr (FLUX == KFVS) {
                 l0 = r[0];
@@ -441,6 +477,9 @@ __global__ __launch_bounds__(flux2d_max_threads<T>()) void k_flux2d(const Solve2
     const T Uw = T(bcl[1]), Vw = T(bcl[2]), lw = T(bcl[3]);
     const T rotT = (side & 1) ? T(-1) : T(1);
     const int oc = c * RS * E;
+    // FP32 departures: the cell's h, b are Mr + the stored values, and the stored wall flux leaves out
+    // Mr's part x Mr len dt
+    const bool del = sizeof(T) == 4 && S.delta;
     T hv[VPT], bv[VPT], ev[VPT], sm[10];
 #pragma unroll
     for (int m = 0; m < 10; m++) sm[m] = T(0);
@@ -452,9 +491,15 @@ __global__ __launch_bounds__(flux2d_max_threads<T>()) void k_flux2d(const Solve2
             const T u = U[iv], v = V[iv], wq = WT[iv];
             const T x = u * n1T + v * n2T, y = v * n1T - u * n2T;
             const T q2 = x * x + y * y;
-            const T h = bload((T*)0, fin, oc + iv * E, 0), bb = bload((T*)0, fin, oc + iv * E, ohb);
+            T h = bload((T*)0, fin, oc + iv * E, 0), bb = bload((T*)0, fin, oc + iv * E, ohb);
             hv[k] = h;
             bv[k] = bb;
+            if constexpr (sizeof(T) == 4) {
+                if (del) {
+                    h += S.mref[iv];
+                    bb += S.mref[nv + iv];
+                }
+            }
             if (!(x * rotT >= T(0))) {
                 sm[0] += wq * x * h;
                 sm[1] += wq * (x * x) * h;
@@ -496,7 +541,13 @@ __global__ __launch_bounds__(flux2d_max_threads<T>()) void k_flux2d(const Solve2
             const T x = u * n1T + v * n2T;
             const bool in = x * rotT >= T(0);
             const T H = AT * ev[k];
-            const T hW = in ? H : hv[k], bW = in ? H * kbT : bv[k];
+            T hW = in ? H : hv[k], bW = in ? H * kbT : bv[k];
+            if constexpr (sizeof(T) == 4) {
+                if (del) {
+                    hW = in ? H - S.mref[iv] : hv[k];
+                    bW = in ? H * kbT - S.mref[nv + iv] : bv[k];
+                }
+            }
             bstore(x * hW * sc, ffo, offf + iv * E, 0);
             bstore(x * bW * sc, ffo, offf + iv * E, ohb);
         }
@@ -509,6 +560,7 @@ __global__ __launch_bounds__(flux2d_max_threads<T>()) void k_flux2d(const Solve2
         fwout[1] = l1 * n1 - l2 * n2;
         fwout[2] = l1 * n2 + l2 * n1;
         fwout[3] = (0.5 * m3 + 0.5 * m4) * len * dt;
+        if (del) (dir == 0 ? S.xcref : S.ycref)[fi] = len * dt;
     }
 }
 
@@ -517,7 +569,7 @@ __global__ __launch_bounds__(flux2d_max_threads<T>()) void k_flux2d(const Solve2
 // wavefront 0 runs the FP64 chain (w += sum fw / ds, conserve_prim, tau) and broadcasts it, then
 // f = (f + (ff_L - ff_R + ff_D - ff_U) / ds + (dt / tau) M) / (1 + dt / tau).  Cells that are not
 // stepped only clear their residual row.
-enum { C_RT = 0, C_INV, C_RDS, C_AN, C_LN, C_UN, C_VN, C_KBN, C_PREU, C_PREV, C_AO, C_LO, C_UO, C_VO, C_KBO, C_N };
+enum { C_RT = 0, C_INV, C_RDS, C_AN, C_LN, C_UN, C_VN, C_KBN, C_PREU, C_PREV, C_AO, C_LO, C_UO, C_VO, C_KBO, C_CU, C_CV, C_N };
 
 template <class T>
 __global__ __launch_bounds__(step2d_max_threads<T>()) void k_cell2d(const Solve2D* __restrict__ solves, const int* __restrict__ wg0,
@@ -555,6 +607,7 @@ __global__ __launch_bounds__(step2d_max_threads<T>()) void k_cell2d(const Solve2
     const T* __restrict__ WT = (const T*)S.wt;
     const double K = S.K, dt = S.dt;
     const bool shak = S.coll == SHAKHOV;
+    const bool del = sizeof(T) == 4 && S.delta;
     double po[4];
 #pragma unroll
     for (int m = 0; m < 4; m++) po[m] = S.prim[(size_t)c * 4 + m];
@@ -567,7 +620,13 @@ __global__ __launch_bounds__(step2d_max_threads<T>()) void k_cell2d(const Solve2
             const int iv = tid + NT * k;
             if (iv < nv) {
                 const T cu = U[iv] - Uo, cv = V[iv] - Vo, c2 = cu * cu + cv * cv, wq = WT[iv];
-                const T hh = h[iv], bb = bd[iv];
+                T hh = h[iv], bb = bd[iv];
+                if constexpr (sizeof(T) == 4) {
+                    if (del) {
+                        hh += S.mref[iv];
+                        bb += S.mref[nv + iv];
+                    }
+                }
                 a[0] += wq * cu * c2 * hh;
                 a[1] += wq * cu * bb;
                 a[2] += wq * cv * c2 * hh;
@@ -603,6 +662,14 @@ __global__ __launch_bounds__(step2d_max_threads<T>()) void k_cell2d(const Solve2
             cst[C_UN] = pn[1];
             cst[C_VN] = pn[2];
             cst[C_KBN] = K / (2.0 * pn[3]);
+            if (del) {
+                // the reference Maxwellian's share of the four face fluxes: sum of +-c (n1 u + n2 v) / ds
+                const double *gL = S.xgeo + (size_t)L * 3, *gR = S.xgeo + (size_t)R * 3;
+                const double *gD = S.ygeo + (size_t)D * 3, *gU = S.ygeo + (size_t)Up * 3;
+                const double cL = S.xcref[L], cR = S.xcref[R], cD = S.ycref[D], cU = S.ycref[Up];
+                cst[C_CU] = (cL * gL[0] - cR * gR[0] + cD * gD[0] - cU * gU[0]) / ds;
+                cst[C_CV] = (cL * gL[1] - cR * gR[1] + cD * gD[1] - cU * gU[1]) / ds;
+            }
             if (shak) {
                 double q[4];
                 for (int m = 0; m < 4; m++) {
@@ -635,6 +702,7 @@ __global__ __launch_bounds__(step2d_max_threads<T>()) void k_cell2d(const Solve2
     }
     __syncthreads();
     const T rT = T(cst[C_RT]), inv = T(cst[C_INV]), rds = T(cst[C_RDS]);
+    const T Cu = del ? T(cst[C_CU]) : T(0), Cv = del ? T(cst[C_CV]) : T(0);
     const T An = T(cst[C_AN]), ln = T(cst[C_LN]), Un = T(cst[C_UN]), Vn = T(cst[C_VN]), kbn = T(cst[C_KBN]);
     T preu = 0, prev = 0, Ao = 0, lo = 0, Uo = 0, Vo = 0, kbo = 0;
     if (shak) {
@@ -666,10 +734,63 @@ __global__ __launch_bounds__(step2d_max_threads<T>()) void k_cell2d(const Solve2
                 MB += gq * (T(2) * lo * c2 + T(K) - T(3)) * (Mo * kbo);
             }
             const int o = nv + iv;
+            if constexpr (sizeof(T) == 4) {
+                if (del) {
+                    // departures: (d + flux of the departures + Mr's flux share + rT (M - Mr)) / (1 + rT)
+                    const T mh = S.mref[iv], mb = S.mref[o];
+                    const T jr = Cu * u + Cv * v;
+                    h[iv] = (h[iv] + (fL[iv] - fR[iv] + fD[iv] - fU[iv]) * rds + jr * mh + rT * (MH - mh)) * inv;
+                    bd[iv] = (bd[iv] + (fL[o] - fR[o] + fD[o] - fU[o]) * rds + jr * mb + rT * (MB - mb)) * inv;
+                    continue;
+                }
+            }
             h[iv] = (h[iv] + (fL[iv] - fR[iv] + fD[iv] - fU[iv]) * rds + rT * MH) * inv;
             bd[iv] = (bd[iv] + (fL[o] - fR[o] + fD[o] - fU[o]) * rds + rT * MB) * inv;
         }
     }
+}
+
+// departure <-> full values of an FP32 fused solve, in place (grid-stride over cells and faces)
+__global__ __launch_bounds__(256) void k_delta2d(const Solve2D* __restrict__ pS, int mode) {
+    const Solve2D& S = *pS;
+    const int nv = S.nv;
+    const long RS = 2L * nv, C = long(S.nx) * S.ny, FX = long(S.nx + 1) * S.ny, FY = long(S.nx) * (S.ny + 1);
+    const float* __restrict__ mr = S.mref;
+    const float* __restrict__ U = (const float*)S.u;
+    const float* __restrict__ V = (const float*)S.v;
+    const long stride = long(gridDim.x) * blockDim.x, t0 = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    float* f = (float*)S.f;
+    for (long e = t0; e < C * RS; e += stride) {
+        const float m = mr[e % RS];
+        f[e] = mode == 2 ? f[e] + m : f[e] - m;
+    }
+    for (int dir = 0; dir < 2; dir++) {
+        const long nfc = dir == 0 ? FX : FY;
+        float* ff = (float*)(dir == 0 ? S.xff : S.yff);
+        double* cr = dir == 0 ? S.xcref : S.ycref;
+        const double* geo = dir == 0 ? S.xgeo : S.ygeo;
+        if (mode == 2) {
+            for (long e = t0; e < nfc * RS; e += stride) {
+                const long fc = e / RS;
+                const double c = cr[fc];
+                if (c != 0.0) {
+                    const int k = int(e % RS), iv = k % nv;
+                    ff[e] += float(c * (geo[fc * 3] * U[iv] + geo[fc * 3 + 1] * V[iv])) * mr[k];
+                }
+            }
+        } else {
+            // the face fluxes hold full values (as the phase kernels left them): no Mr part is missing
+            for (long e = t0; e < nfc; e += stride) cr[e] = 0.0;
+        }
+    }
+}
+
+hipError_t launch_delta2d(const Solve2D& S, const Solve2D* d_S, int mode, hipStream_t st) {
+    if (!S.delta || (mode != 2 && mode != 3)) return hipErrorInvalidValue;
+    const long n = long(S.nx + 1) * (S.ny + 1) * 2 * S.nv;
+    const int blocks = int(std::min<long>((n + 255) / 256, 4096));
+    hipLaunchKernelGGL(k_delta2d, dim3(blocks), dim3(256), 0, st, d_S, mode);
+    return hipGetLastError();
 }
 
 hipError_t launch_flux2d(bool fp64, int flux, int threads, const Solve2D* d_solves, const int* d_fstart, int nsolves, int nfaces,

@@ -77,6 +77,14 @@ struct Solve2D {
     double* ycoef;                                 // [nx*(ny+1)][kFaceCoef]
     double* rcell;                                 // [nx*ny][16] residual rows
     int* health;                                   // failure flags (kbg_device.h prim_health)
+    // FP32 solves of the fused step store f - Mr, Mr a per-solve reference Maxwellian (h and b), and
+    // the face fluxes of the fused step without Mr's part c x Mr, where x is the normal velocity and c
+    // the face's coefficient of the distribution in its flux (0 for a stored :fix face)
+    int delta;
+    const float* mref;                             // [2][nv] Mr (h, b) as stored in FP32
+    double mmom[11];                               // sum w u^a v^b Mr_h: (a,b) = 10 01 20 11 02 30 21 12 03; Mr_b: 10 01
+    double* xcref;                                 // [(nx+1)*ny] c of each x-face
+    double* ycref;                                 // [nx*(ny+1)]
 };
 
 // doubles per face-coefficient record (a UgksFace2 or a KcuFace<4>, padded)
@@ -135,6 +143,10 @@ hipError_t launch_update1d(bool fp64, const Solve1D& S, const Solve1D* d_S, doub
 hipError_t launch_resid1d(const Solve1D& S, const double* d_rcell, int nrows, int boundary_split, double* d_res,
                           hipStream_t st);
 
+// Conversions of an FP32 fused 2D solve between departures (f - Mr, face fluxes without c x Mr) and
+// the full values, in place: mode 2 departure -> full, mode 3 full -> departure (the face fluxes are
+// then taken as full, c = 0)
+hipError_t launch_delta2d(const Solve2D& S, const Solve2D* d_S, int mode, hipStream_t st);
 hipError_t launch_recon2d(bool fp64, const Solve2D& S, const Solve2D* d_S, hipStream_t st);
 hipError_t launch_evolve2d(bool fp64, const Solve2D& S, const Solve2D* d_S, hipStream_t st);
 hipError_t launch_update2d(bool fp64, const Solve2D& S, const Solve2D* d_S, double* d_rcell, hipStream_t st);
