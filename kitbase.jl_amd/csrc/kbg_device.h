@@ -80,17 +80,20 @@ KBG_D void flag_health(int* h, int bits) {
     if (bits && h) __hip_atomic_fetch_or(h, bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// conserve_prim (theory_macro.jl:185-206) for n = 3 (1D), 4 (2D), 5 (3V)
+// conserve_prim (theory_macro.jl:185-206) for n = 3 (1D), 4 (2D), 5 (3V); 1D and 2D with one
+// division by rho and one for lambda: U = m / rho, lambda = rho / (2 (gamma - 1) (E - m U / 2))
 template <int N> KBG_HD void conserve_prim(const double* W, double gam, double* p) {
     if constexpr (N == 3) {
+        const double r = 1.0 / W[0];
         p[0] = W[0];
-        p[1] = W[1] / W[0];
-        p[2] = 0.5 * W[0] / (gam - 1.0) / (W[2] - 0.5 * (W[1] * W[1]) / W[0]);
+        p[1] = W[1] * r;
+        p[2] = 0.5 * W[0] / ((gam - 1.0) * (W[2] - 0.5 * (W[1] * p[1])));
     } else if constexpr (N == 4) {
+        const double r = 1.0 / W[0];
         p[0] = W[0];
-        p[1] = W[1] / W[0];
-        p[2] = W[2] / W[0];
-        p[3] = 0.5 * W[0] / (gam - 1.0) / (W[3] - 0.5 * (W[1] * W[1] + W[2] * W[2]) / W[0]);
+        p[1] = W[1] * r;
+        p[2] = W[2] * r;
+        p[3] = 0.5 * W[0] / ((gam - 1.0) * (W[3] - 0.5 * (W[1] * p[1] + W[2] * p[2])));
     } else {
         p[0] = W[0];
         p[1] = W[1] / W[0];

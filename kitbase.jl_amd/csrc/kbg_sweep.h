@@ -106,8 +106,11 @@ constexpr int vchunks(int nv) { return nv / kVChunk + 2; }
 struct MaxwCoef {
     float A, nl2, U, kb;  // rho (lambda / pi)^(1/2), -lambda log2(e), U, K / (2 lambda)
 };
+// in FP32 from the FP64 primitive variables (one FP32 square root and reciprocal)
 KBG_D MaxwCoef maxw_coef(const double* p, double K) {
-    return MaxwCoef{float(p[0] * sqrt(p[2] / kPi)), float(-p[2] * 1.4426950408889634074), float(p[1]), float(K / (2.0 * p[2]))};
+    const float lam = float(p[2]);
+    return MaxwCoef{float(p[0]) * __builtin_amdgcn_sqrtf(lam * float(1.0 / kPi)), float(-p[2] * 1.4426950408889634074), float(p[1]),
+                    float(K) * __builtin_amdgcn_rcpf(2.f * lam)};
 }
 // M~ at the 8 nodes of a velocity chunk: one exp2 per node, or on equally spaced nodes the product
 // recurrence M(u + du) = M(u) R, R <- R Q with R = exp(-lambda du (2 (u - U) + du)), Q = exp(-2 lambda du^2).
@@ -577,25 +580,37 @@ KBG_D void sweep1d(const Solve1D& S, const WorkItem& wi, int par, int lane, int 
             conserve_prim<3>(wn, S.gam, pn);
             if (LAST ? upd : upd_mid) flag_health(S.health, prim_health<3>(pn));
             const T rT = relax_ratio<T>(pn[0], pn[2], S.mu, S.om, dt);
-            inv = T(1) / (T(1) + rT);
+            if constexpr (DEL) {
+                // FP32: M~^{n+1} from the same coefficients every kernel evaluates M~ with
+                inv = __builtin_amdgcn_rcpf(1.f + rT);
+                const MaxwCoef mc = maxw_coef(pn, K);
+                A1 = mc.A;
+                nl2 = mc.nl2;
+                Un = mc.U;
+                kbn = NF == 2 ? mc.kb : 0.f;
+            } else {
+                inv = T(1) / (T(1) + rT);
+                A1 = T(pn[0] * sqrt(pn[2] / kPi));
+                nl2 = T(-pn[2] * kLog2e);
+                Un = T(pn[1]);
+                kbn = NF == 2 ? T(K / (2.0 * pn[2])) : T(0);  // the b distribution's factor only
+            }
             rTi = rT * inv;
             // the relaxation weight rT / (1 + rT) is folded into the equilibrium amplitude (and the
             // Shakhov prefactor below), one multiply per velocity less in pass 2
-            A1 = T(pn[0] * sqrt(pn[2] / kPi));
             An = A1 * rTi;
-            nl2 = T(-pn[2] * kLog2e);
-            Un = T(pn[1]);
-            kbn = NF == 2 ? T(K / (2.0 * pn[2])) : T(0);  // the b distribution's factor only
             if (SHAK && shak) {
                 const double q = 0.5 * tot[NRAW];
                 pre = T(0.8 * (1.0 - S.Pr) * (po[2] * po[2]) / po[0] * q) * rTi;
-                // the heat-flux prefactor rides on the old equilibrium's amplitude (pass 2 uses it
-                // only in their product)
-                Ao = T(po[0] * sqrt(po[2] / kPi)) * pre;
-                nlo2 = T(-po[2] * kLog2e);
                 Uo = T(po[1]);
                 lo2 = T(2.0 * po[2]);
                 kbo = NF == 2 ? T(K / (2.0 * po[2])) : T(0);
+                if constexpr (!DEL) {
+                    // the heat-flux prefactor rides on the old equilibrium's amplitude (pass 2 uses it
+                    // only in their product)
+                    Ao = T(po[0] * sqrt(po[2] / kPi)) * pre;
+                    nlo2 = T(-po[2] * kLog2e);
+                }
             }
             if constexpr (!LAST) {
                 // the first of two steps: the cells it updates carry w, prim to the second
