@@ -36,7 +36,7 @@ def errs(pg, po):
 def assert_close(pg, po, l2, linf, label=""):
     e = errs(pg, po)
     print(label, {k: f"{v:.2e}" for k, v in e.items()}, flush=True)
-    bad = {k: v for k, v in e.items() if v > (l2 if k.endswith("_l2") else linf)}
+    bad = {k: v for k, v in e.items() if not v <= (l2 if k.endswith("_l2") else linf)}  # NaN fails
     assert not bad, bad
 
 
