@@ -164,6 +164,19 @@ int kbg_step(kbg_solve *s, double dt, int nsteps, double *residual);
 int kbg_batch_step(kbg_ctx *ctx, kbg_solve *const *solves, int n, const double *dts, int nsteps,
                    double *residuals);
 
+/* timestep (solver_main.jl:240-342) of every solve of a batch from its current state on the device:
+ * per solve i, dts[i] = min(cfl[i] / max over cells of (max(u1[i], |U|) + c) / dx (2D: + the y
+ * term with v1[i]), max_time[i] - sim_time[i]); sim_time / max_time may be NULL (no clip), v1 NULL
+ * in a 1D-only batch.  One reduction per solve, one synchronisation. */
+int kbg_batch_timestep(kbg_ctx *ctx, kbg_solve *const *solves, int n, const double *cfl, const double *u1,
+                       const double *v1, const double *sim_time, const double *max_time, double *dts);
+/* kbg_batch_step with each solve's dt taken from kbg_batch_timestep at the call (per-instance
+ * adaptive dt, recomputed every call; KitBase's solve! keeps the first dt for the whole run):
+ * the dts used are returned in dts[n]. */
+int kbg_batch_step_cfl(kbg_ctx *ctx, kbg_solve *const *solves, int n, const double *cfl, const double *u1,
+                       const double *v1, const double *sim_time, const double *max_time, int nsteps, double *dts,
+                       double *residuals);
+
 /* kbg_batch_step without any device allocation.  No counterpart in KitBase: it is for callers that
  * cannot allocate device memory while they step (code under HIP graph capture, tight serving loops,
  * harnesses that forbid hipMalloc).  Same arguments, kernels and results as kbg_batch_step; the

@@ -78,6 +78,8 @@ _PROTOS = {
     "kbg_batch_step": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_int, _dp, C.c_int, _dp]),
     "kbg_solve_health": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_int]),
     "kbg_batch_step_noalloc": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_int, _dp, C.c_int, _dp]),
+    "kbg_batch_timestep": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_int, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "kbg_batch_step_cfl": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_int, _dp, _dp, _dp, _dp, _dp, C.c_int, _dp, _dp]),
 }
 
 EXPORTED = tuple(_PROTOS)

@@ -1258,6 +1258,37 @@ int kbg_timestep(kbg_solve* s, double cfl, double u1, double v1, double sim_time
     return KBG_OK;
 }
 
+int kbg_batch_timestep(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double* cfl, const double* u1, const double* v1,
+                       const double* sim_time, const double* max_time, double* dts) {
+    if (!ctx || (n > 0 && (!solves || !cfl || !u1 || !dts))) return fail(KBG_ERR_ARG, "NULL argument");
+    hipStream_t st = ctx->stream;
+    // one reduction per solve on the context stream, each into its solve's result slot; one sync
+    for (int i = 0; i < n; i++) {
+        kbg_solve* s = solves[i];
+        if (!s || s->ctx != ctx) return fail(KBG_ERR_ARG, "solve " + std::to_string(i) + " is NULL or from another context");
+        if (!(cfl[i] > 0.0)) return fail(KBG_ERR_ARG, "cfl must be positive");
+        if (s->d.dim == 1)
+            HIPCHK(kbg::launch_cfl(s->h1.prim, s->nw, s->d.ng, s->d.nx, s->dx, nullptr, u1[i], 0.0, s->d.gamma, s->res, st));
+        else
+            HIPCHK(kbg::launch_cfl(s->h2.prim, 4, 0, s->N, s->dx, s->dy, u1[i], v1 ? v1[i] : 0.0, s->d.gamma, s->res, st));
+    }
+    std::vector<double> tmax(n, 0.0);
+    for (int i = 0; i < n; i++) HIPCHK(hipMemcpyAsync(&tmax[i], solves[i]->res, 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (int i = 0; i < n; i++) {
+        const double d = cfl[i] / tmax[i];
+        const double left = (max_time ? max_time[i] : INFINITY) - (sim_time ? sim_time[i] : 0.0);
+        dts[i] = d < left ? d : left;
+    }
+    return KBG_OK;
+}
+
+int kbg_batch_step_cfl(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double* cfl, const double* u1, const double* v1,
+                       const double* sim_time, const double* max_time, int nsteps, double* dts, double* residuals) {
+    if (int rc = kbg_batch_timestep(ctx, solves, n, cfl, u1, v1, sim_time, max_time, dts)) return rc;
+    return kbg_batch_step(ctx, solves, n, dts, nsteps, residuals);
+}
+
 int kbg_step(kbg_solve* s, double dt, int nsteps, double* residual) {
     if (!s) return fail(KBG_ERR_ARG, "NULL solve");
     if (nsteps < 0) return fail(KBG_ERR_ARG, "nsteps < 0");

@@ -640,6 +640,45 @@ def batch_step(solves: list[DeviceSolve], dts, nsteps: int, residuals: np.ndarra
         residuals[...] = r
 
 
+def _cfl_args(solves, sim_time, max_time):
+    ks = [s.ks for s in solves]
+    cfl = np.array([k.set.cfl for k in ks], dtype=np.float64)
+    u1 = np.array([k.vs.u1 for k in ks], dtype=np.float64)
+    v1 = np.array([getattr(k.vs, "v1", 0.0) if k.dim == 2 else 0.0 for k in ks], dtype=np.float64)
+    st = np.zeros(len(ks)) if sim_time is None else np.ascontiguousarray(np.broadcast_to(sim_time, (len(ks),)), dtype=np.float64)
+    mt = np.array([k.set.maxTime for k in ks], dtype=np.float64) if max_time is None else \
+        np.ascontiguousarray(np.broadcast_to(max_time, (len(ks),)), dtype=np.float64)
+    return cfl, u1, v1, st, mt
+
+
+def batch_timestep(solves: list[DeviceSolve], sim_time=None, max_time=None, ctx: Context | None = None) -> np.ndarray:
+    """timestep (solver_main.jl:240-342) of every solve from its device-resident state, one reduction
+    per solve (kbg_batch_timestep): each SolverSet's cfl, velocity bounds and maxTime."""
+    ctx = ctx or solves[0].ctx
+    hs = (C.c_void_p * len(solves))(*[s.handle.value for s in solves])
+    cfl, u1, v1, st, mt = _cfl_args(solves, sim_time, max_time)
+    dts = np.zeros(len(solves))
+    _lib.check(ctx.lib.kbg_batch_timestep(ctx.handle, hs, len(solves), _lib.dptr(cfl), _lib.dptr(u1), _lib.dptr(v1),
+                                          _lib.dptr(st), _lib.dptr(mt), _lib.dptr(dts)))
+    return dts
+
+
+def batch_step_cfl(solves: list[DeviceSolve], nsteps: int, sim_time=None, max_time=None,
+                   residuals: np.ndarray | None = None, ctx: Context | None = None) -> np.ndarray:
+    """kbg_batch_step with each solve's dt from its own state at the call (per-instance adaptive dt,
+    kbg_batch_step_cfl); returns the dts used."""
+    ctx = ctx or solves[0].ctx
+    hs = (C.c_void_p * len(solves))(*[s.handle.value for s in solves])
+    cfl, u1, v1, st, mt = _cfl_args(solves, sim_time, max_time)
+    dts = np.zeros(len(solves))
+    r = None if residuals is None else np.zeros((len(solves), 5))
+    _lib.check(ctx.lib.kbg_batch_step_cfl(ctx.handle, hs, len(solves), _lib.dptr(cfl), _lib.dptr(u1), _lib.dptr(v1),
+                                          _lib.dptr(st), _lib.dptr(mt), nsteps, _lib.dptr(dts), _lib.dptr(r)))
+    if residuals is not None:
+        residuals[...] = r
+    return dts
+
+
 # KitBase-named entry points (reconstruct!, evolve!, update!, solve!) on device-resident state
 
 

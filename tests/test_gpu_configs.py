@@ -171,3 +171,45 @@ def test_checkpoint_resume_against_oracle(gpu_ctx, case, tmp_path):
     assert_close(cb.prim, co.prim, tol, tol, case)
     if prec == "fp64":
         assert np.max(np.abs(cb.f - co.f)) <= 1e-9 * np.max(np.abs(co.f))
+
+
+# ---------------------------------------------------------------- per-instance adaptive dt in a batch
+
+def test_batch_timestep_and_adaptive_batch_step(gpu_ctx):
+    """kbg_batch_timestep gives every solve the host timestep formula (solver_main.jl:240-342) of its
+    current state, and kbg_batch_step_cfl steps each solve with its own dt: the same as stepping it
+    alone with that dt, bit for bit."""
+    cfgs = [kb.configs.riemann_1d1f1v(i) for i in kb.configs.batch_instances(5, seed=3)]
+    cfgs += [kb.configs.sod_1d1f1v(), kb.configs.cavity_2d2f2v(nx=10, ny=9, nu=8, nv=8)]
+    runs = []
+    for cfg in cfgs:
+        ks = kb.SolverSet(cfg)
+        ctr, face = kb.init_fvm(ks)
+        runs.append((ks, ctr, kb.DeviceSolve(ks, ctr, face, precision="fp32", ctx=gpu_ctx),
+                     kb.DeviceSolve(ks, copy.deepcopy(ctr), face, precision="fp32", ctx=gpu_ctx)))
+    devs = [r[2] for r in runs]
+    # the first call is KitBase's dt of the initial state (the sod case: 7.427e-4)
+    dts0 = kb.batch_timestep(devs)
+    for (ks, ctr, _, _), d in zip(runs, dts0):
+        assert d == pytest.approx(kb.timestep(ks, ctr), rel=1e-14)
+    assert dts0[5] == pytest.approx(7.427e-4, rel=1e-3)
+    for call in range(3):
+        used = kb.batch_step_cfl(devs, 4)
+        # the state moved: the dt of each solve follows its own state
+        for (ks, ctr, a, b), d in zip(runs, used):
+            b.step(d, 4)
+            ca, cb = copy.deepcopy(ctr), copy.deepcopy(ctr)
+            a.download(ca)
+            b.download(cb)
+            assert np.array_equal(ca.f, cb.f) and np.array_equal(ca.w, cb.w), (ks.set.space, call)
+        after = kb.batch_timestep(devs)
+        for (ks, ctr, a, _), d in zip(runs, after):
+            c = copy.deepcopy(ctr)
+            a.download(c)
+            assert d == pytest.approx(kb.timestep(ks, c), rel=1e-12)
+    # max_time clips the last step
+    clipped = kb.batch_timestep(devs, sim_time=0.05, max_time=0.05 + 1e-6)
+    assert np.allclose(clipped, 1e-6, rtol=1e-9)
+    for r in runs:
+        r[2].close()
+        r[3].close()
