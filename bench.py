@@ -255,14 +255,17 @@ def main():
     devs = [kb.DeviceSolve(ks, ctr, face, precision="fp32", ctx=ctx) for ks, ctr, face, _ in inst]
     log(f"[bench] fp32 upload {time.time() - t0:.1f}s")
     tuned, taken = None, 0
+    vw_now, graphs_now = 0, False
     if a.seg:
         ctx.set_schedule(a.seg, 0)
     elif not a.no_tune:
         best, trials, taken = kb.autotune.tune(ctx, devs, dts)
-        tuned = {"best": {"seg_cells": best[0], "streams": best[1]}, "tuner_steps": taken,
-                 "ms_per_step": {f"seg{k[0]}_streams{k[1]}": round(v, 4) for k, v in trials.items()}}
-        log(f"[bench] autotuner: seg_cells {best[0]}, streams {best[1]} ({taken} trial steps): "
-            + ", ".join(f"{k}: {v:.3f} ms" for k, v in trials.items()))
+        vw_now, graphs_now = best[2], best[3]
+        tuned = {"best": {"seg_cells": best[0], "streams": best[1], "sweep_vw": best[2], "graphs": best[3]},
+                 "tuner_steps": taken,
+                 "ms_per_step": {kb.autotune.key_str(k): round(v, 4) for k, v in trials.items()}}
+        log(f"[bench] autotuner: {kb.autotune.key_str(best)} ({taken} trial steps): "
+            + ", ".join(f"{kb.autotune.key_str(k)}: {v:.3f} ms" for k, v in trials.items()))
     if W:
         kb.batch_step(devs, dts, W, ctx=ctx)
     ctx.sync()
@@ -278,6 +281,8 @@ def main():
         d.close()
     devs = None
     schedule_now = ctx.schedule()
+    ctx.set_graphs(False)  # the other legs run eager, full-width (their results do not depend on it)
+    ctx.set_sweep_vw(0)
 
     # ---- accuracy at t = 0.2: FP32 batch against the FP64 single-solve path, every instance
     nts = t_end_steps(inst)
@@ -374,7 +379,8 @@ def main():
 
     ms_step = dev_ms / K
     achieved = alg_bytes_step / (ms_step * 1e-3) / 1e9
-    sched = f"seg{schedule_now[0]}_streams{schedule_now[1] or 4}_steps{a.sweep_steps}"
+    sched = (f"seg{schedule_now[0]}_streams{schedule_now[1] or 4}_steps{a.sweep_steps}" + (f"_vw{vw_now}" if vw_now else "")
+             + ("_graphs" if graphs_now else ""))
     traffic, tsched = profile_traffic(points, sched)
     out = {
         "metric": "cell×velocity updates/s, FP32-batched vs FP64 plain @1e-4 L2; p99 solve latency",
@@ -406,7 +412,9 @@ def main():
                      "traffic_schedule": tsched,
                      "kernel": f"k_sweep1d<float,1F,KFVS>: {a.sweep_steps} time step(s) per launch, one launch "
                                f"per stream ({schedule_now[1] or 4} concurrent streams, {schedule_now[0]} cells "
-                               "per segment); achieved = per-update algorithmic bytes x updates / device time",
+                               f"per segment, {vw_now or 'full-width'} velocities per wavefront, "
+                               f"{'HIP graph replay' if graphs_now else 'eager launches'}); "
+                               "achieved = per-update algorithmic bytes x updates / device time",
                      "limiter": "VALU issue (the per-velocity arithmetic of two time steps per pass over f), "
                                 "not HBM bandwidth: see profiles/sweep1d_fp32_batch512_pmc.txt",
                      "schedule": sched, "steps_per_launch": a.sweep_steps,

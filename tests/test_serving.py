@@ -133,17 +133,28 @@ def test_serving_mix_is_reproducible():
 
 
 def test_autotuner_picks_the_fastest_and_caches(tmp_path):
-    """The tuner with a fake context: the candidate with the smallest step time wins and is cached."""
-    times = {(60, 4): 0.2, (60, 2): 0.15, (30, 4): 0.3}
+    """The tuner with a fake context: schedules first, then velocities per wavefront at the best
+    schedule, then graph replay; the fastest candidate wins, is left set and is cached."""
+    times = {(60, 4, 0, False): 0.2, (60, 2, 0, False): 0.15, (30, 4, 0, False): 0.3,
+             (60, 2, 32, False): 0.14, (60, 2, 16, False): 0.16, (60, 2, 32, True): 0.145}
 
     class Ctx:
-        sched = (60, 4)
+        sched, vw, graphs = (60, 4), 0, False
 
         def set_schedule(self, a, b):
             self.sched = (a, b)
 
+        def set_sweep_vw(self, v):
+            self.vw = v
+
+        def set_graphs(self, g):
+            self.graphs = g
+
+        def state(self):
+            return (*self.sched, self.vw, self.graphs)
+
         def last_steps_ms(self):
-            return times[self.sched] * 6
+            return times[self.state()] * 6
 
     class Dev:
         def __init__(self, nv):
@@ -156,11 +167,13 @@ def test_autotuner_picks_the_fastest_and_caches(tmp_path):
     try:
         ctx = Ctx()
         devs = [Dev(32), Dev(64)]
-        best, res, taken = kb.autotune.tune(ctx, devs, [1e-3, 1e-3], candidates=list(times), cache=tmp_path / "t.json")
-        assert best == (60, 2) and ctx.sched == (60, 2) and taken == sum(calls)
-        ctx.sched = (30, 4)
-        best2, _, taken2 = kb.autotune.tune(ctx, devs, [1e-3, 1e-3], candidates=list(times), cache=tmp_path / "t.json")
-        assert best2 == (60, 2) and taken2 == 0 and ctx.sched == (60, 2)
+        cands = [(60, 4), (60, 2), (30, 4)]
+        best, res, taken = kb.autotune.tune(ctx, devs, [1e-3, 1e-3], candidates=cands, cache=tmp_path / "t.json")
+        assert best == (60, 2, 32, False) and ctx.state() == best and taken == sum(calls)
+        assert set(res) == set(times)
+        ctx.set_schedule(30, 4)
+        best2, _, taken2 = kb.autotune.tune(ctx, devs, [1e-3, 1e-3], candidates=cands, cache=tmp_path / "t.json")
+        assert best2 == best and taken2 == 0 and ctx.state() == best
     finally:
         kb.solver.batch_step = orig
 
