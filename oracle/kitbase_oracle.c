@@ -1736,3 +1736,131 @@ void orc_set_num_threads(int n) {
     (void)n;
 #endif
 }
+
+/* ------------------------------------------------------------------------------------------ */
+/* Unstructured triangle meshes                                                                 */
+/* ------------------------------------------------------------------------------------------ */
+
+/* evolve!   src/Solver/solver_evolution.jl:872-1035.  KitBase's unstructured evolve! calls the 2D
+ * Maxwell wall without its `rot` argument (flux_boundary.jl:160, :190 take one), which has no
+ * method; the face normal of a boundary face points into the mesh, so the wall is taken with
+ * rot = 1 (incoming velocities vn > 0 carry the wall Maxwellian). */
+int orc_evolve_us(orc_solverus *S, double dt) {
+    const int nv = S->nv, nf = S->nf, RS = nf * nv;
+    int bad = 0;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int f = 0; f < S->nface; f++) {
+        if (S->fkind[f] == 2) continue;
+        double *vn = (double *)malloc(2 * nv * sizeof(double)), *vt = vn + nv;
+        const double n1 = S->fn[2 * f], n2 = S->fn[2 * f + 1], len = S->flen[f];
+        for (int i = 0; i < nv; i++) {
+            vn[i] = S->u[i] * n1 + S->v[i] * n2;
+            vt[i] = S->v[i] * n1 - S->u[i] * n2;
+        }
+        double fw[4];
+        double *ffo = S->ff + (size_t)f * RS;
+        if (S->fkind[f] == 0) {
+            const double *a = S->f + (size_t)S->fc[2 * f] * RS, *b = S->f + (size_t)S->fc[2 * f + 1] * RS;
+            if (nf == 1) orc_flux_kfvs_1f2v(fw, ffo, a, b, vn, vt, S->weights, nv, dt, len, NULL, NULL);
+            else orc_flux_kfvs_2f2v(fw, ffo, ffo + nv, a, a + nv, b, b + nv, vn, vt, S->weights, nv, dt, len, NULL, NULL, NULL, NULL);
+        } else {
+            const int c = S->fc[2 * f] >= 0 ? S->fc[2 * f] : S->fc[2 * f + 1];
+            if (c < 0) {
+                bad = 1;
+                free(vn);
+                continue;
+            }
+            double bc[4];
+            orc_local_frame(S->fwall + 4 * (size_t)f, 4, n1, n2, bc);
+            const double *h = S->f + (size_t)c * RS;
+            if (nf == 1) orc_flux_boundary_maxwell_1f2v(fw, ffo, bc, h, vn, vt, S->weights, nv, dt, len, 1);
+            else orc_flux_boundary_maxwell_2f2v(fw, ffo, ffo + nv, bc, h, h + nv, vn, vt, S->weights, nv, S->K, dt, len, 1);
+        }
+        orc_global_frame(fw, 4, n1, n2, S->fw + 4 * (size_t)f);
+        free(vn);
+    }
+    return bad ? -1 : 0;
+}
+
+/* triangle step!   src/Solver/step_1f.jl:228-285 (BGK: its Shakhov branch reads h and b, which the
+ * one-distribution step does not have), src/Solver/step_2f.jl:256-326 */
+static void orc_step_tri(const orc_solverus *S, int c, double dt, double *RES, double *AVG) {
+    const int nv = S->nv, nf = S->nf, RS = nf * nv;
+    double *w = S->w + 4 * (size_t)c, *prim = S->prim + 4 * (size_t)c, *h = S->f + (size_t)c * RS;
+    double *b = nf == 2 ? h + nv : NULL;
+    const double *u = S->u, *v = S->v, *wt = S->weights, K = S->K, ds = S->area[c];
+    double w_old[4] = {w[0], w[1], w[2], w[3]};
+    double *SH = (double *)calloc(2 * nv, sizeof(double)), *SB = SH + nv;
+    if (nf == 2 && S->coll == ORC_SHAKHOV) {
+        double qa = 0, qb = 0, qc = 0, qd = 0;
+        for (int i = 0; i < nv; i++) {
+            const double cu = u[i] - prim[1], cv = v[i] - prim[2], c2 = cu * cu + cv * cv;
+            qa += wt[i] * cu * c2 * h[i];
+            qb += wt[i] * cu * b[i];
+            qc += wt[i] * cv * c2 * h[i];
+            qd += wt[i] * cv * b[i];
+        }
+        const double q0 = 0.5 * (qa + qb), q1 = 0.5 * (qc + qd);
+        for (int i = 0; i < nv; i++) {
+            const double cu = u[i] - prim[1], cv = v[i] - prim[2], c2 = cu * cu + cv * cv;
+            const double MH = prim[0] * (prim[3] / PI_) * exp(-prim[3] * c2), MB = MH * K / (2.0 * prim[3]);
+            const double pre = 0.8 * (1.0 - S->Pr) * (prim[3] * prim[3]) / prim[0] * (cu * q0 + cv * q1);
+            SH[i] = pre * (2.0 * prim[3] * c2 + K - 5.0) * MH;
+            SB[i] = pre * (2.0 * prim[3] * c2 + K - 3.0) * MB;
+        }
+    }
+    const int *cf = S->cf + 3 * (size_t)c;
+    const double *dr = S->dirc + 3 * (size_t)c;
+    for (int m = 0; m < 4; m++)
+        w[m] -= (S->fw[4 * (size_t)cf[0] + m] * dr[0] + S->fw[4 * (size_t)cf[1] + m] * dr[1] + S->fw[4 * (size_t)cf[2] + m] * dr[2]) / ds;
+    orc_conserve_prim(w, 4, S->gam, prim);
+    for (int m = 0; m < 4; m++) {
+        RES[m] += (w[m] - w_old[m]) * (w[m] - w_old[m]);
+        AVG[m] += fabs(w[m]);
+    }
+    const double tau = orc_vhs_collision_time(prim[0], prim[3], S->mu, S->om);
+    const double *F1 = S->ff + (size_t)cf[0] * RS, *F2 = S->ff + (size_t)cf[1] * RS, *F3 = S->ff + (size_t)cf[2] * RS;
+    for (int i = 0; i < nv; i++) {
+        const double cu = u[i] - prim[1], cv = v[i] - prim[2];
+        double MH = prim[0] * (prim[3] / PI_) * exp(-prim[3] * (cu * cu + cv * cv));
+        double MB = MH * K / (2.0 * prim[3]);
+        MH += SH[i];
+        MB += SB[i];
+        h[i] = (h[i] - (F1[i] * dr[0] + F2[i] * dr[1] + F3[i] * dr[2]) / ds + dt / tau * MH) / (1.0 + dt / tau);
+        if (nf == 2)
+            b[i] = (b[i] - (F1[nv + i] * dr[0] + F2[nv + i] * dr[1] + F3[nv + i] * dr[2]) / ds + dt / tau * MB) / (1.0 + dt / tau);
+    }
+    free(SH);
+}
+
+/* update!   src/Solver/solver_update.jl:359-471, then update_boundary.jl:520-590 (cellType 3) */
+int orc_update_us(orc_solverus *S, double dt, double *res) {
+    if (S->nf == 1 && S->coll == ORC_SHAKHOV) return -1;
+    double RES[4] = {0, 0, 0, 0}, AVG[4] = {0, 0, 0, 0};
+    for (int c = 0; c < S->ncell; c++)
+        if (S->ctype[c] == 0 || S->ctype[c] == 2) orc_step_tri(S, c, dt, RES, AVG);
+    if (res)
+        for (int m = 0; m < 4; m++) res[m] = sqrt(RES[m] * S->ncell) / (AVG[m] + 1.e-7);
+    const int RS = S->nf * S->nv;
+    for (int c = 0; c < S->ncell; c++) {
+        if (S->ctype[c] != 3) continue;
+        int id[2], k = 0;
+        for (int j = 0; j < 3 && k < 2; j++)
+            if (S->nb[3 * (size_t)c + j] != -1) id[k++] = S->nb[3 * (size_t)c + j];
+        if (k < 2) return -1;
+        for (int m = 0; m < 4; m++) S->w[4 * (size_t)c + m] = 0.5 * (S->w[4 * (size_t)id[0] + m] + S->w[4 * (size_t)id[1] + m]);
+        for (int i = 0; i < RS; i++)
+            S->f[(size_t)c * RS + i] = 0.5 * (S->f[(size_t)id[0] * RS + i] + S->f[(size_t)id[1] * RS + i]);
+        orc_conserve_prim(S->w + 4 * (size_t)c, 4, S->gam, S->prim + 4 * (size_t)c);
+    }
+    return 0;
+}
+
+int orc_run_us(orc_solverus *S, double dt, int nsteps, double *res) {
+    for (int t = 0; t < nsteps; t++) {
+        int rc = orc_evolve_us(S, dt);
+        if (!rc) rc = orc_update_us(S, dt, res);
+        if (rc) return rc;
+    }
+    return 0;
+}

@@ -199,6 +199,28 @@ int orc_evolve_2d(orc_solver2d *S, double dt);              /* solver_evolution.
 int orc_update_2d(orc_solver2d *S, double dt, double *res); /* solver_update.jl:193-246, update_boundary.jl:172-306 */
 int orc_run_2d(orc_solver2d *S, double dt, int nsteps, double *res);
 
+/* ---- unstructured triangle meshes (2D1F2V / 2D2F2V, KFVS) ----
+ * evolve! solver_evolution.jl:872-1035, update! solver_update.jl:359-471, triangle step!
+ * step_1f.jl:228-285 / step_2f.jl:256-326, cellType-3 cells update_boundary.jl:520-590.
+ * Face f: cells fc[2f], fc[2f+1] (-1 outside the mesh), unit normal fn[2f..] from the first cell to
+ * the second (into the mesh on a boundary face), length flen[f], kind fkind[f]: 0 interior KFVS
+ * (first order: no reconstruct! for these cells), 1 Maxwell wall (the mesh side's cell has
+ * cellType 2; wall prim fwall[4f..] in the global frame), 2 a stored flux that stays.
+ * Cell c: faces cf[3c..], dirc[3c..] = sign(cell's outward normal . face normal), area, KitBase
+ * cellType ctype[c] (0 and 2 stepped, 3 the mean of its two neighbours nb[3c..], others fixed). */
+typedef struct {
+    int ncell, nface, nv, nf, coll;
+    double K, gam, mu, om, Pr;
+    const double *u, *v, *weights;
+    const int *fc, *fkind, *cf, *ctype, *nb;
+    const double *fn, *flen, *fwall, *dirc, *area;
+    double *w, *prim, *f;   /* ncell x 4, ncell x 4, ncell x nf x nv */
+    double *fw, *ff;        /* nface x 4, nface x nf x nv */
+} orc_solverus;
+int orc_evolve_us(orc_solverus *S, double dt);
+int orc_update_us(orc_solverus *S, double dt, double *res);
+int orc_run_us(orc_solverus *S, double dt, int nsteps, double *res);
+
 int orc_num_threads(void);
 void orc_set_num_threads(int n);
 

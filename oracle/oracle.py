@@ -82,6 +82,18 @@ class Solver2DStruct(C.Structure):
     ]
 
 
+class SolverUSStruct(C.Structure):
+    _fields_ = [
+        ("ncell", C.c_int), ("nface", C.c_int), ("nv", C.c_int), ("nf", C.c_int), ("coll", C.c_int),
+        ("K", C.c_double), ("gam", C.c_double), ("mu", C.c_double), ("om", C.c_double), ("Pr", C.c_double),
+        ("u", _dp), ("v", _dp), ("weights", _dp),
+        ("fc", C.POINTER(C.c_int)), ("fkind", C.POINTER(C.c_int)), ("cf", C.POINTER(C.c_int)),
+        ("ctype", C.POINTER(C.c_int)), ("nb", C.POINTER(C.c_int)),
+        ("fn", _dp), ("flen", _dp), ("fwall", _dp), ("dirc", _dp), ("area", _dp),
+        ("w", _dp), ("prim", _dp), ("f", _dp), ("fw", _dp), ("ff", _dp),
+    ]
+
+
 FLUX = {"kfvs": 0, "kcu": 1, "ugks": 2}
 COLL = {"bgk": 0, "shakhov": 1}
 BC = {"fix": 0, "maxwell": 1}
@@ -165,3 +177,38 @@ def set_threads(n: int):
 
 def num_threads() -> int:
     return int(load().orc_num_threads())
+
+
+class OracleSolveUS:
+    """The oracle's unstructured triangle solve (orc_run_us) over a mesh from kitbase_amd.mesh; the
+    state (an UnstructState) is modified in place."""
+
+    def __init__(self, ks, ps, st, fwall=None, fkind=None):
+        kb = sys.modules["kitbase_amd"]
+        self.lib = load()
+        self.keep = []
+        g = ks.gas
+
+        def k(a, t=np.float64):
+            a = np.ascontiguousarray(np.asarray(a, dtype=t))
+            self.keep.append(a)
+            return a.ctypes.data_as(C.POINTER(C.c_int)) if t == np.int32 else p(a)
+
+        fn, fl = kb.mesh.face_geometry(ps)
+        S = SolverUSStruct()
+        S.ncell, S.nface, S.nv, S.nf, S.coll = ps.ncell, ps.nface, ks.vs.nv, ks.nf, COLL[ks.set.collision]
+        S.K, S.gam, S.mu, S.om, S.Pr = g.K, g.gamma, g.mu_ref, g.omega, g.Pr
+        S.u, S.v, S.weights = k(ks.vs.u.ravel()), k(ks.vs.v.ravel()), k(ks.vs.weights.ravel())
+        S.fc, S.fkind = k(ps.faceCells, np.int32), k(kb.mesh.face_kinds(ps) if fkind is None else fkind, np.int32)
+        S.cf, S.ctype, S.nb = k(ps.cellFaces, np.int32), k(ps.cellType, np.int32), k(ps.cellNeighbors, np.int32)
+        S.fn, S.flen, S.fwall = k(fn), k(fl), k(kb.mesh.wall_prims(ks, ps) if fwall is None else fwall)
+        S.dirc, S.area = k(kb.mesh.cell_dirc(ps, fn)), k(ps.cellArea)
+        S.w, S.prim, S.f, S.fw, S.ff = p(st.w), p(st.prim), p(st.f), p(st.fw), p(st.ff)
+        self.S, self.st = S, st
+
+    def run(self, dt, nsteps, res=None):
+        r = np.zeros(4) if res is None else res
+        rc = self.lib.orc_run_us(C.byref(self.S), C.c_double(dt), C.c_int(nsteps), p(r))
+        if rc != 0:
+            raise RuntimeError(f"oracle orc_run_us returned {rc}")
+        return r

@@ -1071,3 +1071,139 @@ def test_cavity_1f2v_symmetry_and_mass():
         assert np.max(np.abs(f - np.transpose(f, (1, 0, 3, 2)))) <= 1e-11 * np.max(np.abs(f))
         assert np.sum(ctr.w[..., 0] * ks.ps.dx * ks.ps.dy) == pytest.approx(m0, rel=1e-12)
         assert ks.gas.gamma == 2.0
+
+
+# ---------------------------------------------------------------- unstructured triangles (oracle)
+
+def _us_case(cfg, nx=6, ny=5):
+    ks = kb.SolverSet(cfg)
+    ps = kb.mesh.tri_rectangle(0.0, 1.0, nx, 0.0, 1.0, ny)
+    st = kb.mesh.init_fvm_us(ks, ps)
+    return ks, ps, st, kb.mesh.timestep_us(ks, ps, st)
+
+
+def test_us_mesh_closure_and_orientation():
+    """init_fvm's geometry (solver_init.jl:447-492): the cells tile the domain, every cell's faces
+    close (sum of dirc len n = 0 up to KitBase's 1e-6 in unit_normal), interior normals point from
+    the first cell to the second and boundary normals into the mesh."""
+    ps = kb.mesh.tri_rectangle(0.0, 2.0, 7, -1.0, 1.0, 4)
+    fn, fl = kb.mesh.face_geometry(ps)
+    dirc = kb.mesh.cell_dirc(ps, fn)
+    assert ps.cellArea.sum() == pytest.approx(4.0, rel=1e-14)
+    for c in range(ps.ncell):
+        s = sum(dirc[c, j] * fl[ps.cellFaces[c, j]] * fn[ps.cellFaces[c, j]] for j in range(3))
+        assert np.linalg.norm(s) < 1e-5
+    for f in range(ps.nface):
+        a, b = ps.faceCells[f]
+        if a != -1 and b != -1:
+            assert np.dot(fn[f], ps.cellCenter[b] - ps.cellCenter[a]) > 0
+        else:
+            c = a if a != -1 else b
+            assert np.dot(fn[f], ps.cellCenter[c] - ps.faceCenter[f]) > 0
+
+
+@pytest.mark.parametrize("space", ["2d2f2v", "2d1f2v"])
+def test_us_rest_state_is_a_fixed_point(space):
+    """Uniform gas at rest in a box whose walls are at rest at the gas temperature: the face fluxes
+    cancel in each cell and the relaxation returns f (BGK fixed point) -- up to KitBase's unit_normal,
+    whose 1e-6 added to the edge length (geo_general.jl:232) leaves each cell's faces open by
+    ~1e-6 / edge, so the state moves at the 1e-7 level (a wrong sign or frame would move it at O(1))."""
+    cfg = kb.configs.cavity_2d2f2v(nu=8, nv=8, uLid=0.0) if space == "2d2f2v" else kb.configs.cavity_2d1f2v(nu=8, nv=8, uLid=0.0)
+    ks, ps, st, dt = _us_case(cfg)
+    f0, w0 = st.f.copy(), st.w.copy()
+    oracle.OracleSolveUS(ks, ps, st).run(dt, 5)
+    assert np.max(np.abs(st.w - w0)) < 5e-6
+    assert np.max(np.abs(st.f - f0)) < 5e-6 * np.max(f0)
+
+
+@pytest.mark.parametrize("coll", ["bgk", "shakhov"])
+def test_us_mass_is_conserved_with_a_moving_lid(coll):
+    """The Maxwell walls carry no mass (rho_w = -SF/SG) and each interior face enters its two cells
+    with opposite dirc, so the total mass stays at its initial value while the lid drives a flow."""
+    ks, ps, st, dt = _us_case(kb.configs.cavity_2d2f2v(nu=8, nv=8, collision=coll, knudsen=0.1), 8, 8)
+    m0 = float(np.sum(st.w[:, 0] * ps.cellArea))
+    oracle.OracleSolveUS(ks, ps, st).run(dt, 60)
+    assert float(np.sum(st.w[:, 0] * ps.cellArea)) == pytest.approx(m0, rel=1e-13)
+    assert np.max(np.abs(st.prim[:, 1])) > 1e-3
+
+
+def test_us_quarter_turn_of_the_problem_turns_the_solution():
+    """The same cavity turned by 90 degrees about the box centre (points (x, y) -> (1 - y, x); the
+    lid becomes the left wall moving in +y) gives the turned solution: rho, T unchanged, (U, V) ->
+    (-V, U).  Catches a sign or frame slip in the normals, dirc or the wall's local frame."""
+    ks, ps, st, dt = _us_case(kb.configs.cavity_2d2f2v(nu=8, nv=8, knudsen=0.1), 6, 6)
+    pr = copy.deepcopy(ps)
+    pr.points = np.stack([1.0 - ps.points[:, 1], ps.points[:, 0]], axis=1)
+    pr.cellCenter = np.stack([1.0 - ps.cellCenter[:, 1], ps.cellCenter[:, 0]], axis=1)
+    pr.faceCenter = np.stack([1.0 - ps.faceCenter[:, 1], ps.faceCenter[:, 0]], axis=1)
+    fwall = kb.mesh.wall_prims(ks, ps)
+    fwall_r = np.zeros_like(fwall)
+    fwall_r[:, 0], fwall_r[:, 1], fwall_r[:, 2], fwall_r[:, 3] = fwall[:, 0], -fwall[:, 2], fwall[:, 1], fwall[:, 3]
+    sr = copy.deepcopy(st)
+    oracle.OracleSolveUS(ks, ps, st).run(dt, 40)
+    oracle.OracleSolveUS(ks, pr, sr, fwall=fwall_r).run(dt, 40)
+    assert np.max(np.abs(st.prim[:, 1])) > 1e-3
+    np.testing.assert_allclose(sr.prim[:, 0], st.prim[:, 0], rtol=1e-12)
+    np.testing.assert_allclose(sr.prim[:, 3], st.prim[:, 3], rtol=1e-12)
+    np.testing.assert_allclose(sr.prim[:, 1], -st.prim[:, 2], atol=1e-13)
+    np.testing.assert_allclose(sr.prim[:, 2], st.prim[:, 1], atol=1e-13)
+
+
+@pytest.mark.parametrize("space,coll", [("2d2f2v", "bgk"), ("2d1f2v", "bgk"), ("2d2f2v", "shakhov")])
+def test_us_lid_drives_the_same_momentum_and_energy_as_the_structured_solver(space, coll):
+    """The lid's diffuse wall (flux_boundary.jl:160-220) is the same on a triangulated box as on the
+    structured one: over the first steps the x-momentum and energy the lid puts into the gas agree
+    between the two meshes (first-order KFVS on both) within the meshes' difference, with the sign
+    of the lid's motion.  Catches a wall taken from the wrong side, a tangential-velocity sign or a
+    frame slip."""
+    base = dict(nu=12, nv=12, knudsen=0.1, collision=coll, nx=16, ny=16, interpOrder=1, flux="kfvs")
+    cfg = kb.configs.cavity_2d2f2v(**base) if space == "2d2f2v" else kb.configs.cavity_2d1f2v(**base)
+    ks = kb.SolverSet(cfg)
+    ctr, face = kb.init_fvm(ks)
+    ps = kb.mesh.tri_rectangle(0.0, 1.0, 16, 0.0, 1.0, 16)
+    st = kb.mesh.init_fvm_us(ks, ps)
+    dt = min(kb.timestep(ks, ctr), kb.mesh.timestep_us(ks, ps, st))
+    e0 = float(np.sum(st.w[:, 3] * ps.cellArea))
+    oracle.OracleSolveUS(ks, ps, st).run(dt, 30)
+    oracle.OracleSolve(ks, ctr, face).run(dt, 30)
+    area = ks.ps.dx * ks.ps.dy
+    mom_s, mom_u = float(np.sum(ctr.w[..., 1] * area)), float(np.sum(st.w[:, 1] * ps.cellArea))
+    den_s = float(np.sum(ctr.w[..., 3] * area)) - e0
+    den_u = float(np.sum(st.w[:, 3] * ps.cellArea)) - e0
+    assert mom_s > 0 and mom_u > 0
+    assert mom_u == pytest.approx(mom_s, rel=0.05), (mom_u, mom_s)
+    assert den_u == pytest.approx(den_s, rel=0.1), (den_u, den_s)
+
+
+@pytest.mark.parametrize("coll", [0, 1])
+def test_us_triangle_step_relaxes_like_the_structured_step(coll):
+    """With every face flux zero (stored :fix-like faces), the triangle step! (step_2f.jl:256-326)
+    is the structured 2D2F2V step! (step_2f.jl:179-249, pinned above by the Prandtl-rate test):
+    same BGK / Shakhov relaxation of h and b from a distribution that carries heat flux."""
+    ks = kb.SolverSet(kb.configs.cavity_2d2f2v(nu=24, nv=24, vMeshType="rectangle", umin=-8.0, umax=8.0,
+                                               vmin=-8.0, vmax=8.0, collision=("bgk", "shakhov")[coll]))
+    ps = kb.mesh.tri_rectangle(0.0, 1.0, 1, 0.0, 1.0, 1)
+    ps.cellType[:] = 0
+    st = kb.mesh.init_fvm_us(ks, ps)
+    u, v, w = ks.vs.u.ravel(), ks.vs.v.ravel(), ks.vs.weights.ravel()
+    K, gam = ks.gas.K, ks.gas.gamma
+    rho, U, V, lam = 0.9, 0.3, -0.2, 0.8
+    cu, cv = u - U, v - V
+    c2 = cu * cu + cv * cv
+    H = rho * (lam / np.pi) * np.exp(-lam * c2)
+    h0 = H * (1 + 0.05 * cu * (c2 - 2.0 / lam) - 0.03 * cv * (c2 - 2.0 / lam))
+    b0 = H * K / (2 * lam) * (1 + 0.1 * cu - 0.07 * cv)
+    Wm = A([np.dot(w, h0), np.dot(w, u * h0), np.dot(w, v * h0), 0.5 * (np.dot(w, (u * u + v * v) * h0) + np.dot(w, b0))])
+    P = A(np.zeros(4)); call("orc_conserve_prim", Wm, 4, gam, P)
+    for c in range(ps.ncell):
+        st.w[c], st.prim[c], st.f[c, 0], st.f[c, 1] = Wm, P, h0, b0
+    dt = 0.01
+    oracle.OracleSolveUS(ks, ps, st, fkind=np.full(ps.nface, 2)).run(dt, 1)
+    W2, P2, h, b = A(Wm.copy()), A(P.copy()), A(h0.copy()), A(b0.copy())
+    z4, zn = A(np.zeros(4)), A(np.zeros(u.size))
+    call("orc_step_2f2v", W2, P2, h, b, z4, zn, zn, z4, zn, zn, z4, zn, zn, z4, zn, zn, A(u), A(v), A(w), u.size, K, gam,
+         ks.gas.mu_ref, ks.gas.omega, ks.gas.Pr, 0.5, dt, A(np.zeros(4)), A(np.zeros(4)), coll)
+    for c in range(ps.ncell):
+        np.testing.assert_allclose(st.f[c, 0], h, rtol=1e-12, atol=1e-300)
+        np.testing.assert_allclose(st.f[c, 1], b, rtol=1e-12, atol=1e-300)
+        np.testing.assert_allclose(st.w[c], W2, rtol=1e-13)
