@@ -121,6 +121,46 @@ int kbg_solve_destroy(kbg_solve *s);
 /* Device bytes a solve with this description occupies on the fused path (admission control). */
 size_t kbg_solve_bytes(const kbg_desc *desc);
 
+/* An unstructured triangle solve (2D1F2V or 2D2F2V): the fields of KitBase's UnstructPSpace
+ * (geo_unstruct.jl:10-32) and the geometry init_fvm derives from them (solver_init.jl:387-640), with
+ * the VSpace2D / Gas / wall fields its evolve! (solver_evolution.jl:872-1035) and update!
+ * (solver_update.jl:359-471) read.  Cells and faces are 0-based; -1 marks "no cell".  Faces take
+ * KitBase's first-order KFVS flux (flux_kfvs.jl:615-697) or its diffuse Maxwell wall
+ * (flux_boundary.jl:160-220, with rot = 1: the normal of a boundary face points into the mesh).
+ * Host arrays of such a solve: w, prim [ncell][4]; f [ncell][nf][nv]; fw [nface][4] (global frame);
+ * ff [nface][nf][nv].  kbg_upload / kbg_download / kbg_*_faces / kbg_evolve / kbg_update / kbg_step /
+ * kbg_batch_step / kbg_batch_step_noalloc / kbg_solve_health take it like any solve (residuals: 4
+ * values); kbg_reconstruct does nothing (the unstructured evolve! reads no slopes) and the slope
+ * calls fail with KBG_ERR_UNSUPPORTED; kbg_timestep / kbg_batch_timestep take the unstructured
+ * timestep (solver_main.jl:400-426): dt = cfl / max over cells of ((u1 + c)^2 + (v1 + c)^2)^(1/2) /
+ * cell_dmin, clipped to max_time - sim_time. */
+typedef struct kbg_udesc {
+    int nf;        /* 1 (f) or 2 (h, b) */
+    int nv;        /* velocity points, u-fastest; at most 4096 */
+    int ncell, nface;
+    int precision; /* KBG_FP32 or KBG_FP64 */
+    int collision; /* KBG_BGK, or KBG_SHAKHOV with nf = 2 (step_1f.jl's triangle Shakhov reads h and b) */
+    double K, gamma, mu_ref, omega, Pr;
+    const double *u, *v, *weights; /* [nv] */
+    const int *face_cells;         /* [nface][2] ps.faceCells */
+    const int *face_kind;          /* [nface] 0 interior (KFVS); 1 Maxwell wall (a boundary face of a
+                                      cellType-2 cell); 2 a boundary face whose stored flux is kept */
+    const double *face_geo;        /* [nface][3] n1, n2, length (face.n, face.len of init_fvm) */
+    const double *face_wall;       /* [nface][4] wall prim in the global frame (KS.ib.bc at the face
+                                      centre), read for kind-1 faces; NULL when there are none */
+    const int *cell_faces;         /* [ncell][3] ps.cellFaces */
+    const double *cell_dirc;       /* [ncell][3] sign(dot(ctr.n[j], face.n)) of update! (solver_update.jl:375) */
+    const double *cell_area;       /* [ncell] ps.cellArea */
+    const int *cell_type;          /* [ncell] ps.cellType: 0 and 2 stepped, 3 the mean of its first two
+                                      neighbours (update_boundary.jl:520-590), others kept */
+    const int *cell_nb;            /* [ncell][3] ps.cellNeighbors */
+    const double *cell_dmin;       /* [ncell] least of ctr.dx (timestep); NULL: the timestep calls fail */
+} kbg_udesc;
+
+int kbg_usolve_create(kbg_ctx *ctx, const kbg_udesc *desc, kbg_solve **out);
+/* Device bytes an unstructured solve with this description occupies. */
+size_t kbg_usolve_bytes(const kbg_udesc *desc);
+
 /* Failure detection.  No counterpart in KitBase, where a NaN or a negative temperature propagates
  * silently: every cell update checks the cell's new primitive variables, and a cell whose values are
  * not finite (KBG_HEALTH_NONFINITE), whose density is not positive (KBG_HEALTH_RHO) or whose

@@ -47,6 +47,24 @@ class KbgDesc(C.Structure):
     ]
 
 
+_ip = C.POINTER(C.c_int)
+
+
+class KbgUDesc(C.Structure):
+    """Mirror of kbg_udesc (include/kbgpu.h)."""
+
+    _fields_ = [
+        ("nf", C.c_int), ("nv", C.c_int), ("ncell", C.c_int), ("nface", C.c_int),
+        ("precision", C.c_int), ("collision", C.c_int),
+        ("K", C.c_double), ("gamma", C.c_double), ("mu_ref", C.c_double), ("omega", C.c_double),
+        ("Pr", C.c_double),
+        ("u", _dp), ("v", _dp), ("weights", _dp),
+        ("face_cells", _ip), ("face_kind", _ip), ("face_geo", _dp), ("face_wall", _dp),
+        ("cell_faces", _ip), ("cell_dirc", _dp), ("cell_area", _dp), ("cell_type", _ip),
+        ("cell_nb", _ip), ("cell_dmin", _dp),
+    ]
+
+
 # name -> (restype, argtypes)
 _PROTOS = {
     "kbg_version": (C.c_int, []),
@@ -64,6 +82,8 @@ _PROTOS = {
     "kbg_solve_create": (C.c_int, [C.c_void_p, C.POINTER(KbgDesc), C.POINTER(C.c_void_p)]),
     "kbg_solve_destroy": (C.c_int, [C.c_void_p]),
     "kbg_solve_bytes": (C.c_size_t, [C.POINTER(KbgDesc)]),
+    "kbg_usolve_create": (C.c_int, [C.c_void_p, C.POINTER(KbgUDesc), C.POINTER(C.c_void_p)]),
+    "kbg_usolve_bytes": (C.c_size_t, [C.POINTER(KbgUDesc)]),
     "kbg_upload": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
     "kbg_download": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
     "kbg_upload_slopes": (C.c_int, [C.c_void_p, _dp, _dp]),
@@ -122,3 +142,12 @@ def dptr(a: np.ndarray | None):
     if a.dtype != np.float64 or not a.flags["C_CONTIGUOUS"]:
         raise TypeError("expected a C-contiguous float64 array")
     return a.ctypes.data_as(_dp)
+
+
+def iptr(a: np.ndarray | None):
+    """Pointer to a C-contiguous int32 array (or NULL)."""
+    if a is None:
+        return None
+    if a.dtype != np.int32 or not a.flags["C_CONTIGUOUS"]:
+        raise TypeError("expected a C-contiguous int32 array")
+    return a.ctypes.data_as(_ip)

@@ -32,7 +32,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     f"-I{ROOT / 'include'}",
 ]
-SOURCES = ["kbg_sweep_f32.hip", "kbg_sweep_f32b.hip", "kbg_sweep_f32c.hip", "kbg_sweep_f64.hip", "kbg_sweep_f64c.hip", "kbg_1d.hip", "kbg_2d.hip", "kbg_step2d.hip", "kbg_api.hip"]
+SOURCES = ["kbg_sweep_f32.hip", "kbg_sweep_f32b.hip", "kbg_sweep_f32c.hip", "kbg_sweep_f64.hip", "kbg_sweep_f64c.hip", "kbg_1d.hip", "kbg_2d.hip", "kbg_step2d.hip", "kbg_us.hip", "kbg_api.hip"]
 
 
 def _run(cmds: list[list[str]]) -> None:

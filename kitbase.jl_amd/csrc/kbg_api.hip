@@ -115,6 +115,11 @@ struct kbg_solve {
     // 2D
     Solve2D h2{};
     Solve2D* d2 = nullptr;
+    // unstructured triangle solve (kbg_usolve_create; d.dim is 0)
+    bool us = false;
+    kbg::SolveUS hus{};
+    kbg::SolveUS* dus = nullptr;
+    int thru = 0;  // threads per workgroup of its face and cell kernels
     std::vector<void*> allocs;
 };
 
@@ -502,6 +507,22 @@ int push2d(kbg_solve* s) {
     HIPCHK(hipStreamSynchronize(s->ctx->stream));
     return KBG_OK;
 }
+
+int pushus(kbg_solve* s) {
+    HIPCHK(hipMemcpyAsync(s->dus, &s->hus, sizeof(kbg::SolveUS), hipMemcpyHostToDevice, s->ctx->stream));
+    HIPCHK(hipStreamSynchronize(s->ctx->stream));
+    return KBG_OK;
+}
+
+// one evolve! + update! of an unstructured solve on stream st (rows: keep the residual rows)
+int step_us(kbg_solve* s, int rows, hipStream_t st) {
+    HIPCHK(kbg::launch_uface(s->fp64, s->thru, s->hus, s->dus, st));
+    HIPCHK(kbg::launch_ucell(s->fp64, s->thru, s->hus, s->dus, rows, st));
+    HIPCHK(kbg::launch_uavg(s->fp64, s->hus, s->dus, st));
+    return KBG_OK;
+}
+
+constexpr int kUMaxNv = 4096;  // 1024 threads x 4 velocities
 
 // slope / face buffers of the phase-exact path, allocated on first use
 int ensure_phase_buffers(kbg_solve* s) {
@@ -962,6 +983,167 @@ int kbg_solve_create(kbg_ctx* ctx, const kbg_desc* desc, kbg_solve** out) {
     return KBG_OK;
 }
 
+size_t kbg_usolve_bytes(const kbg_udesc* d) {
+    if (!d) return 0;
+    const size_t esz = d->precision == KBG_FP64 ? 8 : 4, C = d->ncell > 0 ? d->ncell : 0, F = d->nface > 0 ? d->nface : 0;
+    const size_t nv = d->nv > 0 ? d->nv : 0, nf = d->nf > 0 ? d->nf : 0;
+    // f and face fluxes; w, prim, dmin, area, dirc, residual rows; fw, geometry, walls; index arrays;
+    // velocity grid
+    return (C + F) * nf * nv * esz + C * (4 + 4 + 1 + 1 + 3 + 8) * 8 + F * (4 + 3 + 4) * 8 + (C * 8 + F * 3) * 4 + 3 * nv * esz +
+           sizeof(kbg::SolveUS) + 16 * 8;
+}
+
+int kbg_usolve_create(kbg_ctx* ctx, const kbg_udesc* ud, kbg_solve** out) {
+    if (!ctx || !ud || !out) return fail(KBG_ERR_ARG, "ctx, desc or out is NULL");
+    const kbg_udesc& u = *ud;
+    if (u.nf != 1 && u.nf != 2) return fail(KBG_ERR_ARG, "nf must be 1 or 2");
+    if (u.nv < 1 || u.nv > kUMaxNv) return fail(KBG_ERR_UNSUPPORTED, "unstructured solves take 1..4096 velocity points");
+    if (u.ncell < 1 || u.nface < 1) return fail(KBG_ERR_ARG, "ncell and nface must be positive");
+    if (u.precision != KBG_FP32 && u.precision != KBG_FP64) return fail(KBG_ERR_ARG, "precision must be KBG_FP32 or KBG_FP64");
+    if (u.collision != KBG_BGK && u.collision != KBG_SHAKHOV) return fail(KBG_ERR_ARG, "unknown collision");
+    if (u.collision == KBG_SHAKHOV && u.nf == 1)
+        return fail(KBG_ERR_UNSUPPORTED, "Shakhov on one distribution: the triangle step of KitBase reads h and b");
+    if (!u.u || !u.v || !u.weights || !u.face_cells || !u.face_kind || !u.face_geo || !u.cell_faces || !u.cell_dirc ||
+        !u.cell_area || !u.cell_type || !u.cell_nb)
+        return fail(KBG_ERR_ARG, "a required unstructured array is NULL");
+    if (!(u.gamma > 1.0) || !(u.mu_ref > 0.0) || !(u.K >= 0.0)) return fail(KBG_ERR_ARG, "gas constants out of range");
+    const size_t C = u.ncell, F = u.nface;
+    // mesh consistency: every index the kernels follow stays inside its array
+    std::vector<int> avg;
+    for (size_t f = 0; f < F; f++) {
+        const int a = u.face_cells[2 * f], b = u.face_cells[2 * f + 1], k = u.face_kind[f];
+        if (a < -1 || a >= (int)C || b < -1 || b >= (int)C) return fail(KBG_ERR_ARG, "face " + std::to_string(f) + ": cell index out of range");
+        if (k == 0 && (a < 0 || b < 0)) return fail(KBG_ERR_ARG, "face " + std::to_string(f) + ": an interior face needs two cells");
+        if (k == 1 && ((a < 0) == (b < 0))) return fail(KBG_ERR_ARG, "face " + std::to_string(f) + ": a wall face needs exactly one cell");
+        if (k < 0 || k > 2) return fail(KBG_ERR_ARG, "face " + std::to_string(f) + ": kind must be 0, 1 or 2");
+        if (k == 1 && !u.face_wall) return fail(KBG_ERR_ARG, "wall faces need face_wall");
+        if (k == 1 && !(u.face_wall[4 * f + 3] > 0.0)) return fail(KBG_ERR_ARG, "face " + std::to_string(f) + ": wall lambda must be positive");
+    }
+    for (size_t c = 0; c < C; c++) {
+        for (int j = 0; j < 3; j++) {
+            const int f = u.cell_faces[3 * c + j], n = u.cell_nb[3 * c + j];
+            if (f < 0 || f >= (int)F) return fail(KBG_ERR_ARG, "cell " + std::to_string(c) + ": face index out of range");
+            if (n < -1 || n >= (int)C) return fail(KBG_ERR_ARG, "cell " + std::to_string(c) + ": neighbour index out of range");
+        }
+        const int t = u.cell_type[c];
+        if ((t == 0 || t == 2) && !(u.cell_area[c] > 0.0)) return fail(KBG_ERR_ARG, "cell " + std::to_string(c) + ": area must be positive");
+        if (t == 3) {
+            int k = 0;
+            for (int j = 0; j < 3; j++) k += u.cell_nb[3 * c + j] != -1;
+            if (k < 2) return fail(KBG_ERR_ARG, "cell " + std::to_string(c) + ": a cellType-3 cell needs two neighbours");
+            avg.push_back((int)c);
+        }
+    }
+    HIPCHK(hipSetDevice(ctx->device));
+    kbg_solve* s = new kbg_solve();
+    s->ctx = ctx;
+    s->us = true;
+    kbg_desc& d = s->d;
+    d.dim = 0;
+    d.nf = u.nf;
+    d.vdim = 2;
+    d.nv = u.nv;
+    d.nx = u.ncell;
+    d.ny = 1;
+    d.precision = u.precision;
+    d.flux = KBG_KFVS;
+    d.collision = u.collision;
+    d.interp_order = 1;
+    d.K = u.K;
+    d.gamma = u.gamma;
+    d.mu_ref = u.mu_ref;
+    d.omega = u.omega;
+    d.Pr = u.Pr;
+    s->hu.assign(u.u, u.u + u.nv);
+    s->hv.assign(u.v, u.v + u.nv);
+    s->hwt.assign(u.weights, u.weights + u.nv);
+    d.u = s->hu.data();
+    d.v = s->hv.data();
+    d.weights = s->hwt.data();
+    s->fp64 = u.precision == KBG_FP64;
+    s->esz = s->fp64 ? 8 : 4;
+    s->nw = 4;
+    s->N = u.ncell;
+    s->RS = u.nf * u.nv;
+    s->thru = 64 * ((u.nv + 255) / 256);
+    auto bail = [&](int code) {
+        kbg_solve_destroy(s);
+        return code;
+    };
+    int rc;
+    void* p;
+    kbg::SolveUS& S = s->hus;
+    S.ncell = u.ncell;
+    S.nface = u.nface;
+    S.nv = u.nv;
+    S.nf = u.nf;
+    S.coll = u.collision == KBG_SHAKHOV ? kbg::SHAKHOV : kbg::BGK;
+    S.navg = (int)avg.size();
+    S.K = u.K;
+    S.gam = u.gamma;
+    S.mu = u.mu_ref;
+    S.om = u.omega;
+    S.Pr = u.Pr;
+    S.dt = 0.0;
+    if ((rc = dev_alloc(s, &s->u, u.nv * s->esz)) || (rc = put_vec(s, s->u, u.u, u.nv))) return bail(rc);
+    if ((rc = dev_alloc(s, &s->v, u.nv * s->esz)) || (rc = put_vec(s, s->v, u.v, u.nv))) return bail(rc);
+    if ((rc = dev_alloc(s, &s->wt, u.nv * s->esz)) || (rc = put_vec(s, s->wt, u.weights, u.nv))) return bail(rc);
+    S.u = s->u;
+    S.v = s->v;
+    S.wt = s->wt;
+    auto put_i = [&](const int*& dst, const int* src, size_t n) -> int {
+        int r;
+        if ((r = dev_alloc(s, &p, n * 4))) return r;
+        if (n) {
+            HIPCHK(hipMemcpyAsync(p, src, n * 4, hipMemcpyHostToDevice, s->ctx->stream));
+            HIPCHK(hipStreamSynchronize(s->ctx->stream));
+        }
+        dst = (const int*)p;
+        return KBG_OK;
+    };
+    auto put_d = [&](const double*& dst, const double* src, size_t n) -> int {
+        int r;
+        if ((r = dev_alloc(s, &p, n * 8))) return r;
+        if (src) {
+            if ((r = put_f64(s, (double*)p, src, n))) return r;
+        } else {
+            HIPCHK(hipMemset(p, 0, n * 8));
+        }
+        dst = (const double*)p;
+        return KBG_OK;
+    };
+    if ((rc = put_i(S.fc, u.face_cells, 2 * F)) || (rc = put_i(S.fkind, u.face_kind, F)) || (rc = put_i(S.cf, u.cell_faces, 3 * C)) ||
+        (rc = put_i(S.ctype, u.cell_type, C)) || (rc = put_i(S.nb, u.cell_nb, 3 * C)) || (rc = put_i(S.avg, avg.data(), avg.size())) ||
+        (rc = put_d(S.fgeo, u.face_geo, 3 * F)) || (rc = put_d(S.fwall, u.face_wall, 4 * F)) || (rc = put_d(S.dirc, u.cell_dirc, 3 * C)) ||
+        (rc = put_d(S.area, u.cell_area, C)))
+        return bail(rc);
+    if (u.cell_dmin) {
+        const double* dm = nullptr;
+        if ((rc = put_d(dm, u.cell_dmin, C))) return bail(rc);
+        s->dx = const_cast<double*>(dm);
+    }
+    struct {
+        void** dst;
+        size_t bytes;
+    } bufs[] = {
+        {(void**)&S.w, C * 4 * 8},          {(void**)&S.prim, C * 4 * 8}, {&S.f, C * s->RS * s->esz},
+        {(void**)&S.fw, F * 4 * 8},         {&S.ff, F * s->RS * s->esz},  {(void**)&s->rcell, C * 8 * 8},
+        {(void**)&s->res, 8 * 8},           {(void**)&S.health, 8},
+    };
+    for (auto& b : bufs) {
+        if ((rc = dev_alloc(s, &p, b.bytes))) return bail(rc);
+        HIPCHK(hipMemset(p, 0, b.bytes));
+        *b.dst = p;
+    }
+    S.rcell = s->rcell;
+    if ((rc = dev_alloc(s, &p, sizeof(kbg::SolveUS)))) return bail(rc);
+    s->dus = (kbg::SolveUS*)p;
+    if ((rc = pushus(s))) return bail(rc);
+    HIPCHK(hipDeviceSynchronize());
+    *out = s;
+    return KBG_OK;
+}
+
 int kbg_solve_destroy(kbg_solve* s) {
     if (!s) return KBG_OK;
     if (s->ctx) (void)hipStreamSynchronize(s->ctx->stream);
@@ -972,12 +1154,16 @@ int kbg_solve_destroy(kbg_solve* s) {
 
 int kbg_solve_health(kbg_solve* s, int* flags, int clear) {
     if (!s || !flags) return fail(KBG_ERR_ARG, "NULL argument");
-    int* h = s->d.dim == 1 ? s->h1.health : s->h2.health;
+    int* h = s->us ? s->hus.health : s->d.dim == 1 ? s->h1.health : s->h2.health;
     HIPCHK(hipMemcpyAsync(flags, h, sizeof(int), hipMemcpyDeviceToHost, s->ctx->stream));
     if (clear) HIPCHK(hipMemsetAsync(h, 0, sizeof(int), s->ctx->stream));
     HIPCHK(hipStreamSynchronize(s->ctx->stream));
     return KBG_OK;
 }
+
+}  // extern "C"
+
+namespace {
 
 // 2D1F2V host arrays [rows][1][nv] <-> the two-distribution device layout [rows][2][nv] (b = 0)
 std::vector<double> widen2d(const kbg_solve* s, const double* src, size_t rows) {
@@ -991,10 +1177,19 @@ void narrow2d(const kbg_solve* s, const std::vector<double>& src, double* dst, s
     for (size_t r = 0; r < rows; r++) std::copy(src.begin() + r * 2 * nv, src.begin() + r * 2 * nv + nv, dst + r * nv);
 }
 
+}  // namespace
+
+extern "C" {
+
 int kbg_upload(kbg_solve* s, const double* w, const double* prim, const double* f) {
     if (!s || !w || !prim || !f) return fail(KBG_ERR_ARG, "NULL argument");
     int rc;
     const size_t N = s->N;
+    if (s->us) {
+        const kbg::SolveUS& S = s->hus;
+        if ((rc = put_f64(s, S.w, w, N * 4)) || (rc = put_f64(s, S.prim, prim, N * 4))) return rc;
+        return put_vec(s, S.f, f, N * s->RS);
+    }
     if (s->d.dim == 1) {
         Solve1D& S = s->h1;
         if ((rc = put_f64(s, S.prim, prim, N * s->nw))) return rc;
@@ -1036,6 +1231,13 @@ int kbg_download(kbg_solve* s, double* w, double* prim, double* f) {
     if (!s) return fail(KBG_ERR_ARG, "NULL solve");
     int rc;
     const size_t N = s->N;
+    if (s->us) {
+        const kbg::SolveUS& S = s->hus;
+        if (w && (rc = get_f64(s, w, S.w, N * 4))) return rc;
+        if (prim && (rc = get_f64(s, prim, S.prim, N * 4))) return rc;
+        if (f && (rc = get_vec(s, f, S.f, N * s->RS))) return rc;
+        return KBG_OK;
+    }
     if (s->d.dim == 1) {
         Solve1D& S = s->h1;
         if (w && (rc = get_f64(s, w, S.w[S.cur], N * s->nw))) return rc;
@@ -1072,6 +1274,7 @@ int kbg_download(kbg_solve* s, double* w, double* prim, double* f) {
 
 int kbg_upload_slopes(kbg_solve* s, const double* sw, const double* sf) {
     if (!s) return fail(KBG_ERR_ARG, "NULL solve");
+    if (s->us) return fail(KBG_ERR_UNSUPPORTED, "an unstructured solve keeps no slopes (its evolve! is first order)");
     int rc;
     const size_t N = s->N;
     if (s->d.dim == 1) {
@@ -1089,6 +1292,7 @@ int kbg_upload_slopes(kbg_solve* s, const double* sw, const double* sf) {
 
 int kbg_download_slopes(kbg_solve* s, double* sw, double* sf) {
     if (!s) return fail(KBG_ERR_ARG, "NULL solve");
+    if (s->us) return fail(KBG_ERR_UNSUPPORTED, "an unstructured solve keeps no slopes (its evolve! is first order)");
     int rc;
     const size_t N = s->N;
     if (s->d.dim == 1) {
@@ -1112,6 +1316,12 @@ int kbg_download_slopes(kbg_solve* s, double* sw, double* sf) {
 int kbg_upload_faces(kbg_solve* s, const double* fw, const double* ff) {
     if (!s) return fail(KBG_ERR_ARG, "NULL solve");
     int rc;
+    if (s->us) {
+        const size_t F = s->hus.nface;
+        if (fw && (rc = put_f64(s, s->hus.fw, fw, F * 4))) return rc;
+        if (ff && (rc = put_vec(s, s->hus.ff, ff, F * s->RS))) return rc;
+        return KBG_OK;
+    }
     if (s->d.dim == 1) {
         if ((rc = ensure_phase_buffers(s))) return rc;
         const size_t nf1 = size_t(s->d.nx) + 1;
@@ -1145,6 +1355,12 @@ int kbg_upload_faces(kbg_solve* s, const double* fw, const double* ff) {
 int kbg_download_faces(kbg_solve* s, double* fw, double* ff) {
     if (!s) return fail(KBG_ERR_ARG, "NULL solve");
     int rc;
+    if (s->us) {
+        const size_t F = s->hus.nface;
+        if (fw && (rc = get_f64(s, fw, s->hus.fw, F * 4))) return rc;
+        if (ff && (rc = get_vec(s, ff, s->hus.ff, F * s->RS))) return rc;
+        return KBG_OK;
+    }
     if (s->d.dim == 1) {
         if ((rc = ensure_phase_buffers(s))) return rc;
         const size_t nf1 = size_t(s->d.nx) + 1;
@@ -1189,6 +1405,7 @@ int kbg_download_faces(kbg_solve* s, double* fw, double* ff) {
 int kbg_reconstruct(kbg_solve* s) {
     if (!s) return fail(KBG_ERR_ARG, "NULL solve");
     int rc;
+    if (s->us) return KBG_OK;  // the unstructured evolve! reads no slopes
     if (s->d.dim == 1) {
         if ((rc = ensure_phase_buffers(s)) || (rc = push1d(s)) || (rc = to_full_rep(s))) return rc;
         HIPCHK(kbg::launch_recon1d(s->fp64, s->h1, s->d1, s->ctx->stream));
@@ -1203,6 +1420,13 @@ int kbg_reconstruct(kbg_solve* s) {
 int kbg_evolve(kbg_solve* s, double dt) {
     if (!s) return fail(KBG_ERR_ARG, "NULL solve");
     int rc;
+    if (s->us) {
+        s->hus.dt = dt;
+        if ((rc = pushus(s))) return rc;
+        HIPCHK(kbg::launch_uface(s->fp64, s->thru, s->hus, s->dus, s->ctx->stream));
+        HIPCHK(hipStreamSynchronize(s->ctx->stream));
+        return KBG_OK;
+    }
     if (s->d.dim == 1) {
         if ((rc = ensure_phase_buffers(s))) return rc;
         s->h1.dt = dt;
@@ -1223,6 +1447,19 @@ int kbg_evolve(kbg_solve* s, double dt) {
 int kbg_update(kbg_solve* s, double dt, double* residual) {
     if (!s) return fail(KBG_ERR_ARG, "NULL solve");
     int rc;
+    if (s->us) {
+        s->hus.dt = dt;
+        if ((rc = pushus(s))) return rc;
+        hipStream_t st = s->ctx->stream;
+        HIPCHK(kbg::launch_ucell(s->fp64, s->thru, s->hus, s->dus, residual ? 1 : 0, st));
+        HIPCHK(kbg::launch_uavg(s->fp64, s->hus, s->dus, st));
+        if (residual) {
+            HIPCHK(kbg::launch_uresid(s->rcell, s->hus.ncell, s->res, st));
+            HIPCHK(hipMemcpyAsync(residual, s->res, 4 * 8, hipMemcpyDeviceToHost, st));
+        }
+        HIPCHK(hipStreamSynchronize(st));
+        return KBG_OK;
+    }
     if (s->d.dim == 1) {
         if ((rc = ensure_phase_buffers(s))) return rc;
         s->h1.dt = dt;
@@ -1243,7 +1480,10 @@ int kbg_timestep(kbg_solve* s, double cfl, double u1, double v1, double sim_time
     if (!s || !dt) return fail(KBG_ERR_ARG, "NULL argument");
     if (!(cfl > 0.0)) return fail(KBG_ERR_ARG, "cfl must be positive");
     hipStream_t st = s->ctx->stream;
-    if (s->d.dim == 1) {
+    if (s->us) {
+        if (!s->dx) return fail(KBG_ERR_ARG, "the unstructured solve was created without cell_dmin");
+        HIPCHK(kbg::launch_ucfl(s->hus.prim, s->dx, s->hus.ncell, u1, v1, s->d.gamma, s->res, st));
+    } else if (s->d.dim == 1) {
         const Solve1D& S = s->h1;
         HIPCHK(kbg::launch_cfl(S.prim, s->nw, s->d.ng, s->d.nx, s->dx, nullptr, u1, 0.0, s->d.gamma, s->res, st));
     } else {
@@ -1267,7 +1507,10 @@ int kbg_batch_timestep(kbg_ctx* ctx, kbg_solve* const* solves, int n, const doub
         kbg_solve* s = solves[i];
         if (!s || s->ctx != ctx) return fail(KBG_ERR_ARG, "solve " + std::to_string(i) + " is NULL or from another context");
         if (!(cfl[i] > 0.0)) return fail(KBG_ERR_ARG, "cfl must be positive");
-        if (s->d.dim == 1)
+        if (s->us) {
+            if (!s->dx) return fail(KBG_ERR_ARG, "solve " + std::to_string(i) + ": unstructured, created without cell_dmin");
+            HIPCHK(kbg::launch_ucfl(s->hus.prim, s->dx, s->hus.ncell, u1[i], v1 ? v1[i] : 0.0, s->d.gamma, s->res, st));
+        } else if (s->d.dim == 1)
             HIPCHK(kbg::launch_cfl(s->h1.prim, s->nw, s->d.ng, s->d.nx, s->dx, nullptr, u1[i], 0.0, s->d.gamma, s->res, st));
         else
             HIPCHK(kbg::launch_cfl(s->h2.prim, 4, 0, s->N, s->dx, s->dy, u1[i], v1 ? v1[i] : 0.0, s->d.gamma, s->res, st));
@@ -1413,6 +1656,11 @@ int batch_step_impl(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double*
     // solves stepped by the phase kernels: buffers and descriptors before the streams fork
     for (int i : others) {
         kbg_solve* s = solves[i];
+        if (s->us) {
+            s->hus.dt = dts[i];
+            if ((rc = pushus(s))) return rc;
+            continue;
+        }
         if (noalloc && !(s->d.dim == 1 ? s->h1.sw != nullptr : s->h2.sf != nullptr))
             return fail(KBG_ERR_STATE, "kbg_batch_step_noalloc: a solve stepped by the phase kernels has no slope and face "
                                        "buffers yet (one kbg_step, kbg_batch_step or phase-level call allocates them)");
@@ -1629,7 +1877,14 @@ int batch_step_impl(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double*
             kbg_solve* s = solves[i];
             hipStream_t ss = sst[stream_of[i]];
             double* r = (last && residuals) ? residuals + 5 * (size_t)i : nullptr;
-            if (s->d.dim == 1) {
+            if (s->us) {
+                if ((e = step_us(s, r ? 1 : 0, ss))) return e;
+                if (r) {
+                    HIPCHK(kbg::launch_uresid(s->rcell, s->hus.ncell, s->res, ss));
+                    HIPCHK(hipMemcpyAsync(r, s->res, 4 * 8, hipMemcpyDeviceToHost, ss));
+                    HIPCHK(hipStreamSynchronize(ss));
+                }
+            } else if (s->d.dim == 1) {
                 HIPCHK(kbg::launch_recon1d(s->fp64, s->h1, s->d1, ss));
                 HIPCHK(kbg::launch_evolve1d(s->fp64, s->h1, s->d1, ss));
                 HIPCHK(kbg::launch_wall1d(s->fp64, s->h1, s->d1, 0, 0, ss));
@@ -1683,10 +1938,14 @@ int batch_step_impl(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double*
         }
         for (int i : others) {
             const kbg_solve* s = solves[i];
-            const void* ids[] = {s, s->d1, s->d2};
+            const void* ids[] = {s, s->d1, s->d2, s->dus};
             put(ids, sizeof ids);
             puti(stream_of[i]);
-            if (s->d.dim == 1) {
+            if (s->us) {
+                kbg::SolveUS h = s->hus;
+                h.dt = 0.0;  // read from the device descriptor, pushed by every call
+                put(&h, sizeof h);
+            } else if (s->d.dim == 1) {
                 Solve1D h = s->h1;
                 h.cur = 0;
                 put(&h, sizeof h);

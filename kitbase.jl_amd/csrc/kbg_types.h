@@ -87,6 +87,35 @@ struct Solve2D {
     double* ycref;                                 // [nx*(ny+1)]
 };
 
+// An unstructured triangle solve (2D1F2V / 2D2F2V, KFVS): faces with their two cells, normal (from
+// the first cell to the second, into the mesh on a boundary face; KitBase's unit_normal) and
+// length; cells with their three faces, dirc signs and area (solver_evolution.jl:872-1035,
+// solver_update.jl:359-471).
+struct SolveUS {
+    int ncell, nface, nv, nf, coll, navg;
+    double K, gam, mu, om, Pr, dt;
+    const int* fc;        // [nface][2] cells (-1 outside)
+    const int* fkind;     // [nface] 0 KFVS, 1 Maxwell wall, 2 stored flux
+    const double* fgeo;   // [nface][3] n1, n2, len
+    const double* fwall;  // [nface][4] wall prim, global frame
+    const int* cf;        // [ncell][3] faces
+    const double* dirc;   // [ncell][3]
+    const double* area;   // [ncell]
+    const int* ctype;     // [ncell] KitBase cellType (0, 2 stepped)
+    const int* nb;        // [ncell][3] neighbours (-1 outside)
+    const int* avg;       // [navg] cellType-3 cells in index order
+    const void* u;        // [nv]
+    const void* v;
+    const void* wt;
+    double* w;            // [ncell][4]
+    double* prim;
+    void* f;              // [ncell][nf][nv]
+    double* fw;           // [nface][4] global frame
+    void* ff;             // [nface][nf][nv]
+    double* rcell;        // [ncell][8] residual rows (sum of squared change, sum of |w|)
+    int* health;
+};
+
 // doubles per face-coefficient record (a UgksFace2 or a KcuFace<4>, padded)
 constexpr int kFaceCoef = 32;
 // Threads per workgroup of the fused 2D step (one face or cell per workgroup): 64 ceil(nv / 256), 4
@@ -147,6 +176,13 @@ hipError_t launch_resid1d(const Solve1D& S, const double* d_rcell, int nrows, in
 // the full values, in place: mode 2 departure -> full, mode 3 full -> departure (the face fluxes are
 // then taken as full, c = 0)
 hipError_t launch_delta2d(const Solve2D& S, const Solve2D* d_S, int mode, hipStream_t st);
+// unstructured triangles: face fluxes (evolve!), cell update (update!) and the cellType-3 means
+hipError_t launch_uface(bool fp64, int threads, const SolveUS& S, const SolveUS* d_S, hipStream_t st);
+hipError_t launch_ucell(bool fp64, int threads, const SolveUS& S, const SolveUS* d_S, int rows, hipStream_t st);
+hipError_t launch_uavg(bool fp64, const SolveUS& S, const SolveUS* d_S, hipStream_t st);
+hipError_t launch_uresid(const double* d_rows, int nrows, double* d_out, hipStream_t st);
+hipError_t launch_ucfl(const double* d_prim, const double* d_dmin, int n, double u1, double v1, double gam, double* d_out,
+                       hipStream_t st);
 hipError_t launch_recon2d(bool fp64, const Solve2D& S, const Solve2D* d_S, hipStream_t st);
 hipError_t launch_evolve2d(bool fp64, const Solve2D& S, const Solve2D* d_S, hipStream_t st);
 hipError_t launch_update2d(bool fp64, const Solve2D& S, const Solve2D* d_S, double* d_rcell, hipStream_t st);

@@ -205,3 +205,137 @@ def timestep_us(ks, ps: UnstructPSpace, st: UnstructState, simTime: float = 0.0)
     tmax = float(np.max(np.sqrt(umax ** 2 + vmax ** 2) / dx.min(axis=1)))
     dt = ks.set.cfl / tmax
     return dt if dt < ks.set.maxTime - simTime else ks.set.maxTime - simTime
+
+
+def make_udesc(ks, ps: UnstructPSpace, precision: str = "fp64", fwall: np.ndarray | None = None,
+               fkind: np.ndarray | None = None):
+    """kbg_udesc for an unstructured solve; returns the struct and the arrays it points into (keep
+    them alive through kbg_usolve_create)."""
+    from . import _lib
+
+    keep: list = []
+
+    def d(a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        keep.append(a)
+        return _lib.dptr(a)
+
+    def i(a):
+        a = np.ascontiguousarray(a, dtype=np.int32)
+        keep.append(a)
+        return _lib.iptr(a)
+
+    u = _lib.KbgUDesc()
+    u.nf, u.nv, u.ncell, u.nface = ks.nf, ks.vs.nv, ps.ncell, ps.nface
+    u.precision = _lib.KBG_FP64 if precision == "fp64" else _lib.KBG_FP32
+    u.collision = _lib.COLL[ks.set.collision]
+    g = ks.gas
+    u.K, u.gamma, u.mu_ref, u.omega, u.Pr = g.K, g.gamma, g.mu_ref, g.omega, g.Pr
+    u.u, u.v, u.weights = d(ks.vs.u.ravel()), d(ks.vs.v.ravel()), d(ks.vs.weights.ravel())
+    fn, fl = face_geometry(ps)
+    u.face_cells = i(ps.faceCells)
+    u.face_kind = i(face_kinds(ps) if fkind is None else fkind)
+    u.face_geo = d(np.column_stack([fn, fl]))
+    u.face_wall = d(wall_prims(ks, ps) if fwall is None else fwall)
+    u.cell_faces = i(ps.cellFaces)
+    u.cell_dirc = d(cell_dirc(ps, fn))
+    u.cell_area = d(ps.cellArea)
+    u.cell_type = i(ps.cellType)
+    u.cell_nb = i(ps.cellNeighbors)
+    u.cell_dmin = d(cell_dx(ps).min(axis=1))
+    return u, keep
+
+
+class DeviceSolveUS:
+    """An unstructured triangle solve resident in HBM (kbg_usolve_create): the device counterpart of
+    the ctr / face arrays init_fvm builds for an UnstructPSpace (solver_init.jl:387-640), stepped by
+    evolve! (solver_evolution.jl:872-1035) and update! (solver_update.jl:359-471)."""
+
+    def __init__(self, ks, ps: UnstructPSpace, st: UnstructState, precision: str = "fp64", ctx=None,
+                 fwall: np.ndarray | None = None, fkind: np.ndarray | None = None):
+        import ctypes as C
+
+        from . import _lib
+        from .solver import default_context
+
+        self.ks, self.ps = ks, ps
+        self.ctx = ctx or default_context()
+        self.lib = self.ctx.lib
+        self.precision = precision
+        desc, keep = make_udesc(ks, ps, precision, fwall, fkind)
+        h = C.c_void_p()
+        _lib.check(self.lib.kbg_usolve_create(self.ctx.handle, C.byref(desc), C.byref(h)))
+        self.handle = h
+        self.upload(st, faces=True)
+
+    def upload(self, st: UnstructState, faces: bool = False):
+        from . import _lib
+
+        _lib.check(self.lib.kbg_upload(self.handle, _lib.dptr(st.w), _lib.dptr(st.prim), _lib.dptr(st.f)))
+        if faces:
+            _lib.check(self.lib.kbg_upload_faces(self.handle, _lib.dptr(st.fw), _lib.dptr(st.ff)))
+
+    def download(self, st: UnstructState, faces: bool = False):
+        from . import _lib
+
+        _lib.check(self.lib.kbg_download(self.handle, _lib.dptr(st.w), _lib.dptr(st.prim), _lib.dptr(st.f)))
+        if faces:
+            _lib.check(self.lib.kbg_download_faces(self.handle, _lib.dptr(st.fw), _lib.dptr(st.ff)))
+
+    def evolve(self, dt: float):
+        from . import _lib
+
+        _lib.check(self.lib.kbg_evolve(self.handle, dt))
+
+    def update(self, dt: float, residual: np.ndarray | None = None):
+        from . import _lib
+
+        _lib.check(self.lib.kbg_update(self.handle, dt, _lib.dptr(residual) if residual is not None else None))
+
+    def step(self, dt: float, nsteps: int = 1, residual: np.ndarray | None = None):
+        from . import _lib
+
+        r = None if residual is None else np.zeros(5)
+        _lib.check(self.lib.kbg_step(self.handle, dt, nsteps, _lib.dptr(r)))
+        if residual is not None:
+            residual[...] = r[: residual.size]
+
+    def timestep(self, simTime: float = 0.0) -> float:
+        """timestep for unstructured control volumes (solver_main.jl:400-426) from the device prim."""
+        import ctypes as C
+
+        from . import _lib
+
+        ks = self.ks
+        dt = C.c_double()
+        _lib.check(self.lib.kbg_timestep(self.handle, ks.set.cfl, ks.vs.u1, ks.vs.v1, simTime, ks.set.maxTime, C.byref(dt)))
+        return dt.value
+
+    def health(self, clear: bool = False) -> int:
+        import ctypes as C
+
+        from . import _lib
+
+        flags = C.c_int(0)
+        _lib.check(self.lib.kbg_solve_health(self.handle, C.byref(flags), 1 if clear else 0))
+        return flags.value
+
+    def close(self):
+        if self.handle:
+            self.lib.kbg_solve_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def usolve_bytes(ks, ps: UnstructPSpace, precision: str = "fp32") -> int:
+    import ctypes as C
+
+    from . import _lib
+
+    d, _keep = make_udesc(ks, ps, precision)
+    return int(_lib.load().kbg_usolve_bytes(C.byref(d)))

@@ -212,3 +212,15 @@ class OracleSolveUS:
         if rc != 0:
             raise RuntimeError(f"oracle orc_run_us returned {rc}")
         return r
+
+    def evolve(self, dt):
+        rc = self.lib.orc_evolve_us(C.byref(self.S), C.c_double(dt))
+        if rc != 0:
+            raise RuntimeError(f"oracle orc_evolve_us returned {rc}")
+
+    def update(self, dt, res=None):
+        r = np.zeros(4) if res is None else res
+        rc = self.lib.orc_update_us(C.byref(self.S), C.c_double(dt), p(r))
+        if rc != 0:
+            raise RuntimeError(f"oracle orc_update_us returned {rc}")
+        return r

@@ -34,12 +34,12 @@ def test_binding_covers_the_header():
     assert set(declared_functions()) == set(kb._lib.EXPORTED)
 
 
-def test_desc_mirror_field_order():
-    body = HEADER[HEADER.index("typedef struct kbg_desc"):]
-    body = body[: body.index("} kbg_desc;")]
+@pytest.mark.parametrize("name,mirror", [("kbg_desc", "KbgDesc"), ("kbg_udesc", "KbgUDesc")])
+def test_desc_mirror_field_order(name, mirror):
+    body = HEADER[HEADER.index(f"typedef struct {name} "):]
+    body = body[: body.index(f"}} {name};")]
     fields = re.findall(r"([a-zA-Z_][a-zA-Z0-9_]*)\s*(?:\[\d+\])?\s*[,;]", re.sub(r"/\*.*?\*/", "", body, flags=re.S))
-    mirror = [f[0] for f in kb._lib.KbgDesc._fields_]
-    assert fields == mirror
+    assert fields == [f[0] for f in getattr(kb._lib, mirror)._fields_]
 
 
 def test_no_gpu_means_an_error_not_a_fallback():
