@@ -514,13 +514,15 @@ int pushus(kbg_solve* s) {
     return KBG_OK;
 }
 
-// one evolve! + update! of an unstructured solve on stream st (rows: keep the residual rows)
-int step_us(kbg_solve* s, int rows, hipStream_t st) {
-    HIPCHK(kbg::launch_uface(s->fp64, s->thru, s->hus, s->dus, st));
-    HIPCHK(kbg::launch_ucell(s->fp64, s->thru, s->hus, s->dus, rows, st));
-    HIPCHK(kbg::launch_uavg(s->fp64, s->hus, s->dus, st));
+// one fused evolve! + update! of an unstructured solve on stream st, reading f from the buffer of
+// relative parity par and writing the other (rows: keep the residual rows)
+int step_us(kbg_solve* s, int par, int rows, hipStream_t st) {
+    HIPCHK(kbg::launch_ustep(s->fp64, s->thru, s->hus, s->dus, par, rows, st));
+    HIPCHK(kbg::launch_uavg(s->fp64, s->hus, s->dus, par + 1, st));
     return KBG_OK;
 }
+
+void* us_f(const kbg_solve* s) { return s->hus.cur ? s->hus.f1 : s->hus.f; }
 
 constexpr int kUMaxNv = 4096;  // 1024 threads x 4 velocities
 
@@ -989,7 +991,7 @@ size_t kbg_usolve_bytes(const kbg_udesc* d) {
     const size_t nv = d->nv > 0 ? d->nv : 0, nf = d->nf > 0 ? d->nf : 0;
     // f and face fluxes; w, prim, dmin, area, dirc, residual rows; fw, geometry, walls; index arrays;
     // velocity grid
-    return (C + F) * nf * nv * esz + C * (4 + 4 + 1 + 1 + 3 + 8) * 8 + F * (4 + 3 + 4) * 8 + (C * 8 + F * 3) * 4 + 3 * nv * esz +
+    return (2 * C + F) * nf * nv * esz + C * (4 + 4 + 1 + 1 + 3 + 8) * 8 + F * (4 + 3 + 4) * 8 + (C * 8 + F * 3) * 4 + 3 * nv * esz +
            sizeof(kbg::SolveUS) + 16 * 8;
 }
 
@@ -1065,7 +1067,7 @@ int kbg_usolve_create(kbg_ctx* ctx, const kbg_udesc* ud, kbg_solve** out) {
     s->nw = 4;
     s->N = u.ncell;
     s->RS = u.nf * u.nv;
-    s->thru = 64 * ((u.nv + 255) / 256);
+    s->thru = kbg::ustep_threads(u.nv);
     auto bail = [&](int code) {
         kbg_solve_destroy(s);
         return code;
@@ -1085,6 +1087,7 @@ int kbg_usolve_create(kbg_ctx* ctx, const kbg_udesc* ud, kbg_solve** out) {
     S.om = u.omega;
     S.Pr = u.Pr;
     S.dt = 0.0;
+    S.cur = 0;
     if ((rc = dev_alloc(s, &s->u, u.nv * s->esz)) || (rc = put_vec(s, s->u, u.u, u.nv))) return bail(rc);
     if ((rc = dev_alloc(s, &s->v, u.nv * s->esz)) || (rc = put_vec(s, s->v, u.v, u.nv))) return bail(rc);
     if ((rc = dev_alloc(s, &s->wt, u.nv * s->esz)) || (rc = put_vec(s, s->wt, u.weights, u.nv))) return bail(rc);
@@ -1126,7 +1129,7 @@ int kbg_usolve_create(kbg_ctx* ctx, const kbg_udesc* ud, kbg_solve** out) {
         void** dst;
         size_t bytes;
     } bufs[] = {
-        {(void**)&S.w, C * 4 * 8},          {(void**)&S.prim, C * 4 * 8}, {&S.f, C * s->RS * s->esz},
+        {(void**)&S.w, C * 4 * 8},          {(void**)&S.prim, C * 4 * 8}, {&S.f, C * s->RS * s->esz}, {&S.f1, C * s->RS * s->esz},
         {(void**)&S.fw, F * 4 * 8},         {&S.ff, F * s->RS * s->esz},  {(void**)&s->rcell, C * 8 * 8},
         {(void**)&s->res, 8 * 8},           {(void**)&S.health, 8},
     };
@@ -1188,7 +1191,7 @@ int kbg_upload(kbg_solve* s, const double* w, const double* prim, const double* 
     if (s->us) {
         const kbg::SolveUS& S = s->hus;
         if ((rc = put_f64(s, S.w, w, N * 4)) || (rc = put_f64(s, S.prim, prim, N * 4))) return rc;
-        return put_vec(s, S.f, f, N * s->RS);
+        return put_vec(s, us_f(s), f, N * s->RS);
     }
     if (s->d.dim == 1) {
         Solve1D& S = s->h1;
@@ -1235,7 +1238,7 @@ int kbg_download(kbg_solve* s, double* w, double* prim, double* f) {
         const kbg::SolveUS& S = s->hus;
         if (w && (rc = get_f64(s, w, S.w, N * 4))) return rc;
         if (prim && (rc = get_f64(s, prim, S.prim, N * 4))) return rc;
-        if (f && (rc = get_vec(s, f, S.f, N * s->RS))) return rc;
+        if (f && (rc = get_vec(s, f, us_f(s), N * s->RS))) return rc;
         return KBG_OK;
     }
     if (s->d.dim == 1) {
@@ -1452,7 +1455,7 @@ int kbg_update(kbg_solve* s, double dt, double* residual) {
         if ((rc = pushus(s))) return rc;
         hipStream_t st = s->ctx->stream;
         HIPCHK(kbg::launch_ucell(s->fp64, s->thru, s->hus, s->dus, residual ? 1 : 0, st));
-        HIPCHK(kbg::launch_uavg(s->fp64, s->hus, s->dus, st));
+        HIPCHK(kbg::launch_uavg(s->fp64, s->hus, s->dus, 0, st));
         if (residual) {
             HIPCHK(kbg::launch_uresid(s->rcell, s->hus.ncell, s->res, st));
             HIPCHK(hipMemcpyAsync(residual, s->res, 4 * 8, hipMemcpyDeviceToHost, st));
@@ -1846,7 +1849,9 @@ int batch_step_impl(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double*
     // solves allow it takes both in one pass, the other units are launched nst times); each sweep
     // launch reads the f / w buffers of its own parity (launches so far), last marks the round whose
     // residual rows are kept
-    auto enqueue_round = [&](int nst, bool last_round) -> int {
+    // ubase: the parity of the round's first step among the call's steps (unstructured solves flip
+    // their f buffer every step)
+    auto enqueue_round = [&](int nst, bool last_round, int ubase) -> int {
         int e;
         for (Launch& L : launches) {
             const int per = L.two ? nst : 1;  // steps per launch
@@ -1878,7 +1883,7 @@ int batch_step_impl(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double*
             hipStream_t ss = sst[stream_of[i]];
             double* r = (last && residuals) ? residuals + 5 * (size_t)i : nullptr;
             if (s->us) {
-                if ((e = step_us(s, r ? 1 : 0, ss))) return e;
+                if ((e = step_us(s, (ubase + ks) & 1, r ? 1 : 0, ss))) return e;
                 if (r) {
                     HIPCHK(kbg::launch_uresid(s->rcell, s->hus.ncell, s->res, ss));
                     HIPCHK(hipMemcpyAsync(r, s->res, 4 * 8, hipMemcpyDeviceToHost, ss));
@@ -1943,7 +1948,8 @@ int batch_step_impl(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double*
             puti(stream_of[i]);
             if (s->us) {
                 kbg::SolveUS h = s->hus;
-                h.dt = 0.0;  // read from the device descriptor, pushed by every call
+                h.dt = 0.0;  // dt and the buffer base are read from the device descriptor, pushed by every call
+                h.cur = 0;
                 put(&h, sizeof h);
             } else if (s->d.dim == 1) {
                 Solve1D h = s->h1;
@@ -1959,8 +1965,8 @@ int batch_step_impl(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double*
                 HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
                 for (Launch& L : launches) L.flips = L.two ? which >> 1 : 0;
                 int e = fork();
-                if (!e) e = enqueue_round((which & 1) || !two ? 1 : 2, false);
-                if (!e && !(which & 1) && !two) e = enqueue_round(1, false);
+                if (!e) e = enqueue_round((which & 1) || !two ? 1 : 2, false, 0);
+                if (!e && !(which & 1) && !two) e = enqueue_round(1, false, 1);
                 if (!e) e = join();
                 hipGraph_t g = nullptr;
                 const hipError_t ce = hipStreamEndCapture(st, &g);
@@ -1986,7 +1992,7 @@ int batch_step_impl(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double*
         if ((rc = fork())) return rc;
         for (int t = 0; t < nsteps;) {
             const int nst = two && nsteps - t >= 2 ? 2 : 1;
-            if ((rc = enqueue_round(nst, t + nst == nsteps))) return rc;
+            if ((rc = enqueue_round(nst, t + nst == nsteps, t & 1))) return rc;
             t += nst;
         }
         if ((rc = join())) return rc;
@@ -2020,6 +2026,8 @@ int batch_step_impl(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double*
         kbg_solve* s = solves[i];
         s->h1.cur = (s->h1.cur + launches[launch_of[i]].flips) & 1;
     }
+    for (int i : others)
+        if (solves[i]->us) solves[i]->hus.cur = (solves[i]->hus.cur + nsteps) & 1;
     // fused 2D solves: residuals from the last step's rows
     if (residuals) {
         for (int i : fused2) {

@@ -109,7 +109,9 @@ struct SolveUS {
     const void* wt;
     double* w;            // [ncell][4]
     double* prim;
-    void* f;              // [ncell][nf][nv]
+    void* f;              // [ncell][nf][nv], buffer 0
+    void* f1;             // buffer 1: the fused step writes the buffer it does not read
+    int cur;              // buffer holding the state at the start of a call (kernels add their parity)
     double* fw;           // [nface][4] global frame
     void* ff;             // [nface][nf][nv]
     double* rcell;        // [ncell][8] residual rows (sum of squared change, sum of |w|)
@@ -177,9 +179,13 @@ hipError_t launch_resid1d(const Solve1D& S, const double* d_rcell, int nrows, in
 // then taken as full, c = 0)
 hipError_t launch_delta2d(const Solve2D& S, const Solve2D* d_S, int mode, hipStream_t st);
 // unstructured triangles: face fluxes (evolve!), cell update (update!) and the cellType-3 means
+// unstructured kernels; par is the step's buffer parity relative to S.cur, threads that of
+// ustep_threads(nv) (the kernels' velocities per thread follow from nv)
+int ustep_threads(int nv);
 hipError_t launch_uface(bool fp64, int threads, const SolveUS& S, const SolveUS* d_S, hipStream_t st);
 hipError_t launch_ucell(bool fp64, int threads, const SolveUS& S, const SolveUS* d_S, int rows, hipStream_t st);
-hipError_t launch_uavg(bool fp64, const SolveUS& S, const SolveUS* d_S, hipStream_t st);
+hipError_t launch_ustep(bool fp64, int threads, const SolveUS& S, const SolveUS* d_S, int par, int rows, hipStream_t st);
+hipError_t launch_uavg(bool fp64, const SolveUS& S, const SolveUS* d_S, int par, hipStream_t st);
 hipError_t launch_uresid(const double* d_rows, int nrows, double* d_out, hipStream_t st);
 hipError_t launch_ucfl(const double* d_prim, const double* d_dmin, int n, double u1, double v1, double gam, double* d_out,
                        hipStream_t st);
