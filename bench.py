@@ -29,6 +29,7 @@ Printed (one JSON line on stdout):
 from __future__ import annotations
 
 import argparse
+import re
 import copy
 import json
 import os
@@ -116,7 +117,17 @@ def profile_traffic(points, schedule):
         traffic = float(vals["hbm_bytes_per_step"].split()[0])
     except (KeyError, ValueError):
         return None, None
-    return traffic, vals.get("schedule")
+    sched = vals.get("schedule")
+    return traffic, sched.split()[0] if sched else None
+
+
+def traffic_key(schedule):
+    """The parts of a schedule string that set the sweep's HBM traffic: the segment length, the
+    steps per launch and the velocities per wavefront (the stream count and graph replay only
+    change how the same launches are issued)."""
+    if not schedule:
+        return None
+    return re.sub(r"_streams\d+|_graphs", "", schedule)
 
 
 def run_batch_to(kb, ctx, inst, nts, precision="fp32"):
@@ -167,6 +178,36 @@ def cavity_leg(kb, ctx, steps):
     return cavity
 
 
+def unstructured_leg(kb, ctx, steps, nx=64, nu=28):
+    """Lid-driven cavity on a triangulated unit square (nx^2 rectangles cut into 2 nx^2 triangles,
+    nu^2 velocities, 2D2F2V KFVS + BGK, Maxwell walls) stepped by the fused unstructured step
+    (k_ustep) in FP32 and FP64; device time from the library's step events."""
+    ks = kb.SolverSet(kb.configs.cavity_2d2f2v(nu=nu, nv=nu, knudsen=0.075))
+    ps = kb.mesh.tri_rectangle(0.0, 1.0, nx, 0.0, 1.0, nx)
+    out = {"config": f"2D2F2V cavity on {ps.ncell} triangles ({ps.nface} faces) x {nu}^2 velocities, KFVS + BGK, "
+                     f"Kn 0.075, Maxwell walls", "cells": ps.ncell, "faces": ps.nface, "steps": steps}
+    for prec in ("fp32", "fp64"):
+        st = kb.mesh.init_fvm_us(ks, ps)
+        dt = kb.mesh.timestep_us(ks, ps, st)
+        d = kb.mesh.DeviceSolveUS(ks, ps, st, precision=prec, ctx=ctx)
+        d.step(dt, 2)
+        ctx.sync()
+        t1 = time.perf_counter()
+        d.step(dt, steps)
+        wall_c = time.perf_counter() - t1
+        dev_ms = ctx.last_steps_ms() / steps
+        d.close()
+        pts = ps.ncell * ks.vs.nv
+        esz = 4 if prec == "fp32" else 8
+        alg = 2 * 2 * esz * pts + 2 * 4 * 8 * ps.ncell  # f (h, b) read + write, w read + write
+        out[prec] = {"updates_per_s": pts * steps / wall_c, "ms_per_step": wall_c * 1e3 / steps,
+                     "device_ms_per_step": dev_ms, "algorithmic_bytes_per_step": alg,
+                     "hbm_frac": alg / (dev_ms * 1e-3) / (HBM_PEAK_GBS * 1e9)}
+    log(f"[bench] unstructured: fp32 {out['fp32']['device_ms_per_step']:.3f} ms/step, "
+        f"fp64 {out['fp64']['device_ms_per_step']:.3f} ms/step (device)")
+    return out
+
+
 def serving_leg(kb, ctx, n, rates, chunk, overlap):
     out = {"mix": "75% 1D tubes of the batch (to t = 0.2), 15% lid-driven cavities {32,48,64,96,128}^2 cells x "
                   "{16,24,32,48}^2 velocities (100 steps), 10% 1D1F3V 24 cells x 24^3 velocities (40 steps); FP32",
@@ -199,6 +240,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="target CPU work of the baseline sample")
     ap.add_argument("--cavity-steps", type=int, default=20, help="timed steps of the 2D cavity leg (0 skips it)")
     ap.add_argument("--cavity-only", action="store_true", help="run only the 2D cavity leg (profiling)")
+    ap.add_argument("--us-steps", type=int, default=50, help="timed steps of the unstructured cavity leg (0 skips it)")
+    ap.add_argument("--us-only", action="store_true", help="run only the unstructured cavity leg (profiling)")
     ap.add_argument("--serve", type=int, default=48, metavar="N",
                     help="serving leg: N requests of the mixed stream per arrival rate (0 skips it)")
     ap.add_argument("--serve-rates", default="10,20,50,100,200,500", help="Poisson arrival rates of the serving leg (1/s)")
@@ -236,6 +279,9 @@ def main():
     kb = load_package()
     if a.cavity_only:
         print(json.dumps({"cavity_2d": cavity_leg(kb, kb.Context(0), max(1, a.cavity_steps))}), flush=True)
+        return
+    if a.us_only:
+        print(json.dumps({"unstructured_2d": unstructured_leg(kb, kb.Context(0), max(1, a.us_steps))}), flush=True)
         return
     K, W = a.steps, a.warmup
     t0 = time.time()
@@ -327,6 +373,8 @@ def main():
 
     # ---- 2D lid-driven cavity (64^2 cells x 32^2 velocities, UGKS, Maxwell walls), fused step
     cavity = cavity_leg(kb, ctx, a.cavity_steps) if a.cavity_steps else None
+    # ---- the same cavity on a triangulated mesh (8192 triangles x 28^2 velocities), fused unstructured step
+    unstructured = unstructured_leg(kb, ctx, a.us_steps) if a.us_steps else None
 
     # ---- serving leg: Poisson arrivals of 1D tubes, 2D cavities and 1D1F3V requests, served together
     serve = None
@@ -401,6 +449,7 @@ def main():
                    "points_per_step": points, "parallelism": "single GPU"},
         "schedule": tuned,
         "cavity_2d": cavity,
+        "unstructured_2d": unstructured,
         "fp64_plain": fp64,
         "speedup_vs_fp64_plain": (value / fp64["value"]) if fp64 else None,
         "l2_vs_fp64": acc,
@@ -408,7 +457,7 @@ def main():
         "p99_solve_latency_ms": serve["rates"].get("100.0", {}).get("p99_ms") if serve else None,
         "serving": serve,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic if tsched == sched else None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic if traffic_key(tsched) == traffic_key(sched) else None,
                      "traffic_schedule": tsched,
                      "kernel": f"k_sweep1d<float,1F,KFVS>: {a.sweep_steps} time step(s) per launch, one launch "
                                f"per stream ({schedule_now[1] or 4} concurrent streams, {schedule_now[0]} cells "
