@@ -181,8 +181,8 @@ def cavity_leg(kb, ctx, steps):
 def unstructured_leg(kb, ctx, steps, nx=64, nu=28):
     """Lid-driven cavity on a triangulated unit square (nx^2 rectangles cut into 2 nx^2 triangles,
     nu^2 velocities, 2D2F2V KFVS + BGK, Maxwell walls) stepped by the fused unstructured step
-    (k_ustep) in FP32 and FP64; device time from the library's step events."""
-    ks = kb.SolverSet(kb.configs.cavity_2d2f2v(nu=nu, nv=nu, knudsen=0.075))
+    (k_ustep) in FP32 and FP64, interpOrder 1; device time from the library's step events."""
+    ks = kb.SolverSet(kb.configs.cavity_2d2f2v(nu=nu, nv=nu, knudsen=0.075, interpOrder=1))
     ps = kb.mesh.tri_rectangle(0.0, 1.0, nx, 0.0, 1.0, nx)
     out = {"config": f"2D2F2V cavity on {ps.ncell} triangles ({ps.nface} faces) x {nu}^2 velocities, KFVS + BGK, "
                      f"Kn 0.075, Maxwell walls", "cells": ps.ncell, "faces": ps.nface, "steps": steps}

@@ -130,8 +130,13 @@ size_t kbg_solve_bytes(const kbg_desc *desc);
  * Host arrays of such a solve: w, prim [ncell][4]; f [ncell][nf][nv]; fw [nface][4] (global frame);
  * ff [nface][nf][nv].  kbg_upload / kbg_download / kbg_*_faces / kbg_evolve / kbg_update / kbg_step /
  * kbg_batch_step / kbg_batch_step_noalloc / kbg_solve_health take it like any solve (residuals: 4
- * values); kbg_reconstruct does nothing (the unstructured evolve! reads no slopes) and the slope
- * calls fail with KBG_ERR_UNSUPPORTED; kbg_timestep / kbg_batch_timestep take the unstructured
+ * values).  The unstructured evolve! extrapolates each cell's f to the face centre with the cell's
+ * slopes, which stay zero at interpOrder 1 (reconstruct! returns at once,
+ * solver_reconstruction.jl:809-880): that first-order scheme is what is built, so kbg_reconstruct
+ * does nothing and the slope calls fail with KBG_ERR_UNSUPPORTED.  interpOrder 2 is refused: its
+ * reconstruct! (reconstruct3!/reconstruct4!, reconstruction.jl:136-254) is not built (for 1F it
+ * reads an undefined `ps`; only vanleer and minmod take three slopes; cells whose neighbours' centres
+ * share an x or y coordinate divide by zero).  kbg_timestep / kbg_batch_timestep take the unstructured
  * timestep (solver_main.jl:400-426): dt = cfl / max over cells of ((u1 + c)^2 + (v1 + c)^2)^(1/2) /
  * cell_dmin, clipped to max_time - sim_time. */
 typedef struct kbg_udesc {
@@ -140,6 +145,7 @@ typedef struct kbg_udesc {
     int ncell, nface;
     int precision; /* KBG_FP32 or KBG_FP64 */
     int collision; /* KBG_BGK, or KBG_SHAKHOV with nf = 2 (step_1f.jl's triangle Shakhov reads h and b) */
+    int interp_order; /* Setup.interpOrder: 1.  2 is refused (KBG_ERR_UNSUPPORTED): see kbg_usolve_create */
     double K, gamma, mu_ref, omega, Pr;
     const double *u, *v, *weights; /* [nv] */
     const int *face_cells;         /* [nface][2] ps.faceCells */

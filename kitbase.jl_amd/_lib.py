@@ -55,7 +55,7 @@ class KbgUDesc(C.Structure):
 
     _fields_ = [
         ("nf", C.c_int), ("nv", C.c_int), ("ncell", C.c_int), ("nface", C.c_int),
-        ("precision", C.c_int), ("collision", C.c_int),
+        ("precision", C.c_int), ("collision", C.c_int), ("interp_order", C.c_int),
         ("K", C.c_double), ("gamma", C.c_double), ("mu_ref", C.c_double), ("omega", C.c_double),
         ("Pr", C.c_double),
         ("u", _dp), ("v", _dp), ("weights", _dp),

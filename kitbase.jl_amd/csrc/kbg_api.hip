@@ -1005,6 +1005,9 @@ int kbg_usolve_create(kbg_ctx* ctx, const kbg_udesc* ud, kbg_solve** out) {
     if (u.collision != KBG_BGK && u.collision != KBG_SHAKHOV) return fail(KBG_ERR_ARG, "unknown collision");
     if (u.collision == KBG_SHAKHOV && u.nf == 1)
         return fail(KBG_ERR_UNSUPPORTED, "Shakhov on one distribution: the triangle step of KitBase reads h and b");
+    if (u.interp_order != 1)
+        return fail(KBG_ERR_UNSUPPORTED, "unstructured solves are built for interpOrder 1 (first order: the cells' slopes stay zero); "
+                                         "the second-order reconstruct! for unstructured cells is not built");
     if (!u.u || !u.v || !u.weights || !u.face_cells || !u.face_kind || !u.face_geo || !u.cell_faces || !u.cell_dirc ||
         !u.cell_area || !u.cell_type || !u.cell_nb)
         return fail(KBG_ERR_ARG, "a required unstructured array is NULL");
@@ -1050,7 +1053,7 @@ int kbg_usolve_create(kbg_ctx* ctx, const kbg_udesc* ud, kbg_solve** out) {
     d.precision = u.precision;
     d.flux = KBG_KFVS;
     d.collision = u.collision;
-    d.interp_order = 1;
+    d.interp_order = u.interp_order;
     d.K = u.K;
     d.gamma = u.gamma;
     d.mu_ref = u.mu_ref;
@@ -1277,7 +1280,7 @@ int kbg_download(kbg_solve* s, double* w, double* prim, double* f) {
 
 int kbg_upload_slopes(kbg_solve* s, const double* sw, const double* sf) {
     if (!s) return fail(KBG_ERR_ARG, "NULL solve");
-    if (s->us) return fail(KBG_ERR_UNSUPPORTED, "an unstructured solve keeps no slopes (its evolve! is first order)");
+    if (s->us) return fail(KBG_ERR_UNSUPPORTED, "an unstructured solve keeps no slopes (it is built for interpOrder 1, where they are zero)");
     int rc;
     const size_t N = s->N;
     if (s->d.dim == 1) {
@@ -1295,7 +1298,7 @@ int kbg_upload_slopes(kbg_solve* s, const double* sw, const double* sf) {
 
 int kbg_download_slopes(kbg_solve* s, double* sw, double* sf) {
     if (!s) return fail(KBG_ERR_ARG, "NULL solve");
-    if (s->us) return fail(KBG_ERR_UNSUPPORTED, "an unstructured solve keeps no slopes (its evolve! is first order)");
+    if (s->us) return fail(KBG_ERR_UNSUPPORTED, "an unstructured solve keeps no slopes (it is built for interpOrder 1, where they are zero)");
     int rc;
     const size_t N = s->N;
     if (s->d.dim == 1) {
@@ -1408,7 +1411,7 @@ int kbg_download_faces(kbg_solve* s, double* fw, double* ff) {
 int kbg_reconstruct(kbg_solve* s) {
     if (!s) return fail(KBG_ERR_ARG, "NULL solve");
     int rc;
-    if (s->us) return KBG_OK;  // the unstructured evolve! reads no slopes
+    if (s->us) return KBG_OK;  // interpOrder 1: reconstruct! leaves the (zero) slopes as they are
     if (s->d.dim == 1) {
         if ((rc = ensure_phase_buffers(s)) || (rc = push1d(s)) || (rc = to_full_rep(s))) return rc;
         HIPCHK(kbg::launch_recon1d(s->fp64, s->h1, s->d1, s->ctx->stream));

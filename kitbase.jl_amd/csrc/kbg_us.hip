@@ -2,7 +2,8 @@
 //
 //   k_ustep   the fused time step: one wavefront per cell, lanes striding the velocity grid.  A
 //             stepped cell (cellType 0 or 2) evaluates the fluxes of its three faces itself --
-//             KitBase's first-order KFVS flux between the face's two cells (flux_kfvs.jl:615-697,
+//             KitBase's KFVS flux between the face's two cells at interpOrder 1, where the cells'
+//             slopes are zero and the face values are the cell values (flux_kfvs.jl:615-697,
 //             with the face's normal and tangential velocities, solver_evolution.jl:872-1035) or the
 //             diffuse Maxwell wall of a boundary face (flux_boundary.jl:160-220), the face moments
 //             reduced in FP64 and turned to the global frame (geo_general.jl:6-26) -- and applies the
@@ -72,9 +73,10 @@ KBG_D UFace uface_at(const SolveUS& S, int f) {
     return F;
 }
 
-// One velocity of an interior face (flux_kfvs.jl:615-697 at first order: sf = 0, so
-// (dt u f - dt^2 u^2 sf / 2) len = dt u f len): the upwind h (b) of the first cell (ha) or the
-// second (ho), its flux and the five moment terms.
+// One velocity of an interior face (flux_kfvs.jl:615-697 as the unstructured evolve! calls it: no
+// slope argument, so (dt u f - dt^2 u^2 sf / 2) len = dt u f len; at interpOrder 1 the face values
+// are the cell values): the upwind h (b) of the first cell (ha) or the second (ho), its flux and the
+// five moment terms.
 template <class T>
 KBG_D void kfvs_vel(T u, T v, T wq, T ha, T ho, T ba, T bo, bool two, T dtT, T lenT, T& fh, T& fb, double (&s)[5]) {
     const T d = heav(u);

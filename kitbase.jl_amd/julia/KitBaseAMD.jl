@@ -7,9 +7,11 @@ Julia glue for libkbgpu.so: device-resident KitBase solves stepped on an MI355X 
 the host `ControlVolume` / `Interface` arrays (for `extract_sol`, `write_jld`, plotting).
 
 Supported: 1D1F1V, 1D2F1V and 1D1F3V (`ControlVolume1F/2F` over `eachindex(ps.x)`, ghosts included;
-`Interface1F/2F` faces 1..nx+1), and 2D2F2V without ghost cells (`ControlVolume2F` matrices with
-`a1face` (nx+1) x ny and `a2face` nx x (ny+1)); fluxes :kfvs, :kcu, :ugks; collisions :bgk,
-:shakhov (not 1D1F3V, whose KitBase step! cannot run it, step_1f.jl:91); boundaries :fix, :maxwell.
+`Interface1F/2F` faces 1..nx+1), 2D1F2V and 2D2F2V without ghost cells (`ControlVolume1F/2F`
+matrices with `a1face` (nx+1) x ny and `a2face` nx x (ny+1)); fluxes :kfvs, :kcu, :ugks; collisions
+:bgk, :shakhov (not 1D1F3V, whose KitBase step! cannot run it, step_1f.jl:91, nor one-distribution
+2D); boundaries :fix, :maxwell.  Unstructured triangle meshes (`UnstructPSpace`, `ControlVolumeUS1F/2F`,
+`Interface2D1F/2F`): :kfvs at interpOrder 1, walls from `cellType` 2.
 
 KitBase v0.9.31 fields read here (file:line of the struct that defines each):
 - `KS.set.space, flux, collision, limiter, interpOrder, boundary, cfl, maxTime`  Setup, struct_setup.jl:106-119
@@ -22,10 +24,15 @@ KitBase v0.9.31 fields read here (file:line of the struct that defines each):
   called as evolve_boundary.jl:19,35 and :77,100,123,147 call them            IB1F/IB2F, struct_ib.jl:33-61
 - `ctr[i].w, prim, f` / `h, b` (and `sw, sf` / `sh, sb` for slopes)      ControlVolume1F/2F, struct_ctr.jl:51-58, 94-103
 - `face[k].fw, ff` / `fh, fb`                                             Interface1F/2F, struct_face.jl:37-40, 56-60
+- `KS.ps.cellid, faceCells, cellFaces, cellNeighbors, cellType, cellArea, faceCenter` (unstructured)
+                                                                          UnstructPSpace, geo_unstruct.jl:10-32
+- `ctr[i].n, dx, w, prim, f` / `h, b`; `face[k].n, len, fw, ff` / `fh, fb` (unstructured)
+                                        ControlVolumeUS1F/2F, struct_ctr.jl:1074-1185; Interface2D1F/2F, struct_face.jl:349-400
 """
 module KitBaseAMD
 
 using KitBase
+using LinearAlgebra: dot
 
 export DeviceContext, DeviceSolve, upload!, download!, batch_step!
 
@@ -41,6 +48,17 @@ struct KbgDesc
     u::Ptr{Cdouble}; v::Ptr{Cdouble}; w::Ptr{Cdouble}; weights::Ptr{Cdouble}
     dx::Ptr{Cdouble}; dy::Ptr{Cdouble}; bc_prim::Ptr{Cdouble}
     xgeo::Ptr{Cdouble}; ygeo::Ptr{Cdouble}; xwall::Ptr{Cdouble}; ywall::Ptr{Cdouble}
+end
+
+# kbg_udesc (include/kbgpu.h), field for field
+struct KbgUDesc
+    nf::Cint; nv::Cint; ncell::Cint; nface::Cint
+    precision::Cint; collision::Cint; interp_order::Cint
+    K::Cdouble; gamma::Cdouble; mu_ref::Cdouble; omega::Cdouble; Pr::Cdouble
+    u::Ptr{Cdouble}; v::Ptr{Cdouble}; weights::Ptr{Cdouble}
+    face_cells::Ptr{Cint}; face_kind::Ptr{Cint}; face_geo::Ptr{Cdouble}; face_wall::Ptr{Cdouble}
+    cell_faces::Ptr{Cint}; cell_dirc::Ptr{Cdouble}; cell_area::Ptr{Cdouble}; cell_type::Ptr{Cint}
+    cell_nb::Ptr{Cint}; cell_dmin::Ptr{Cdouble}
 end
 
 const FLUX = Dict(:kfvs => 0, :kcu => 1, :ugks => 2)
@@ -205,6 +223,68 @@ function DeviceSolve(ks::SolverSet, ctr, a1face, a2face; precision=:fp32, ctx=De
     dev = create(ks, precision, ctx)
     dev.dim == 2 || error("1D solves take DeviceSolve(ks, ctr, face)")
     return upload!(dev, ctr, a1face, a2face)
+end
+
+# ---- unstructured triangle meshes (UnstructPSpace, geo_unstruct.jl:10-32)
+
+# 0-based indices, -1 kept for "no cell"
+zb(a) = Cint[x == -1 ? Cint(-1) : Cint(x - 1) for x in a]
+
+"""
+The descriptor of an unstructured solve: ps.faceCells / cellFaces / cellNeighbors (row-major, 0-based),
+face kinds (0 interior, 1 Maxwell wall: a boundary face of a cellType-2 cell, 2 a boundary face whose
+flux evolve! leaves alone, solver_evolution.jl:883-935), face[i].n and len, the wall state
+KS.ib.bc(faceCenter, p) (solver_evolution.jl:906-910), the dirc signs of update! (solver_update.jl:375),
+ps.cellArea, ps.cellType and the least of ctr[i].dx (timestep, solver_main.jl:400-426).
+"""
+function make_udesc(ks::SolverSet, ctr, face, precision, keep)
+    ps, vs, gas = ks.ps, ks.vs, ks.gas
+    keepd(a) = (a = collect(Float64, a); push!(keep, a); pointer(a))
+    keepi(a) = (a = collect(Cint, a); push!(keep, a); pointer(a))
+    nc, nfc = size(ps.cellid, 1), size(ps.faceCells, 1)
+    kind = Cint[]
+    geo = Float64[]
+    wall = zeros(Float64, 4 * nfc)
+    for i in 1:nfc
+        c1, c2 = ps.faceCells[i, 1], ps.faceCells[i, 2]
+        append!(geo, (face[i].n[1], face[i].n[2], face[i].len))
+        if c1 != -1 && c2 != -1
+            push!(kind, 0)
+        elseif ps.cellType[c1 != -1 ? c1 : c2] == 2
+            push!(kind, 1)
+            wall[4i-3:4i] .= ks.ib.bc(ps.faceCenter[i, 1], ps.faceCenter[i, 2], ks.ib.p)
+        else
+            push!(kind, 2)
+        end
+    end
+    dirc = [sign(dot(ctr[i].n[j], face[ps.cellFaces[i, j]].n)) for j in 1:3, i in 1:nc]
+    nf = hasproperty(first(ctr), :h) ? 2 : 1
+    return KbgUDesc(nf, length(vs.u), nc, nfc, precision == :fp64 ? Cint(1) : Cint(0),
+                    lookup(COLL, ks.set.collision, "collision"), ks.set.interpOrder,
+                    gas.K, gas.γ, gas.μᵣ, gas.ω, gas.Pr,
+                    keepd(vec(vs.u)), keepd(vec(vs.v)), keepd(vec(vs.weights)),
+                    keepi(vec(permutedims(zb(ps.faceCells)))), keepi(kind), keepd(geo), keepd(wall),
+                    keepi(vec(permutedims(zb(ps.cellFaces)))), keepd(vec(dirc)), keepd(ps.cellArea),
+                    keepi(ps.cellType), keepi(vec(permutedims(zb(ps.cellNeighbors)))),
+                    keepd([minimum(c.dx) for c in ctr]))
+end
+
+"""
+    DeviceSolve(ks, ctr, face; precision=:fp32, ctx)   # ks.ps an UnstructPSpace
+
+An unstructured triangle solve (2D1F2V / 2D2F2V, interpOrder 1) from init_fvm's ctr and face
+(solver_init.jl:387-640); evolve!, update!, timestep, solve! and step! then work on it as on the others.
+"""
+function DeviceSolve(ks::SolverSet{<:Any,<:KitBase.UnstructPSpace}, ctr, face; precision=:fp32,
+                     ctx=DeviceContext(0))
+    keep = Any[]
+    d = make_udesc(ks, ctr, face, precision, keep)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    GC.@preserve keep check(ccall((:kbg_usolve_create, libkbgpu), Cint, (Ptr{Cvoid}, Ref{KbgUDesc}, Ref{Ptr{Cvoid}}),
+                                  ctx.handle, Ref(d), h))
+    dev = DeviceSolve(h[], ctx, ks, 0, keep)
+    finalizer(s -> ccall((:kbg_solve_destroy, libkbgpu), Cint, (Ptr{Cvoid},), s.handle), dev)
+    return upload!(dev, ctr, face)
 end
 
 cellvec(ctr, field) = reduce(vcat, [flat(getfield(c, field)) for c in ctr])

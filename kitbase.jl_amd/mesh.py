@@ -229,6 +229,7 @@ def make_udesc(ks, ps: UnstructPSpace, precision: str = "fp64", fwall: np.ndarra
     u.nf, u.nv, u.ncell, u.nface = ks.nf, ks.vs.nv, ps.ncell, ps.nface
     u.precision = _lib.KBG_FP64 if precision == "fp64" else _lib.KBG_FP32
     u.collision = _lib.COLL[ks.set.collision]
+    u.interp_order = ks.set.interpOrder
     g = ks.gas
     u.K, u.gamma, u.mu_ref, u.omega, u.Pr = g.K, g.gamma, g.mu_ref, g.omega, g.Pr
     u.u, u.v, u.weights = d(ks.vs.u.ravel()), d(ks.vs.v.ravel()), d(ks.vs.weights.ravel())

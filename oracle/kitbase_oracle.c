@@ -1741,10 +1741,12 @@ void orc_set_num_threads(int n) {
 /* Unstructured triangle meshes                                                                 */
 /* ------------------------------------------------------------------------------------------ */
 
-/* evolve!   src/Solver/solver_evolution.jl:872-1035.  KitBase's unstructured evolve! calls the 2D
- * Maxwell wall without its `rot` argument (flux_boundary.jl:160, :190 take one), which has no
- * method; the face normal of a boundary face points into the mesh, so the wall is taken with
- * rot = 1 (incoming velocities vn > 0 carry the wall Maxwellian). */
+/* evolve!   src/Solver/solver_evolution.jl:872-1035 at interpOrder 1: the face values
+ * f + sf . (x_face - x_cell) are the cell values, the slopes staying zero (reconstruct! returns at
+ * once, solver_reconstruction.jl:809-880).  KitBase's unstructured evolve! calls the 2D Maxwell wall
+ * without its `rot` argument (flux_boundary.jl:160, :190 take one), which has no method; the face
+ * normal of a boundary face points into the mesh, so the wall is taken with rot = 1 (incoming
+ * velocities vn > 0 carry the wall Maxwellian). */
 int orc_evolve_us(orc_solverus *S, double dt) {
     const int nv = S->nv, nf = S->nf, RS = nf * nv;
     int bad = 0;

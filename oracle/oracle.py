@@ -181,9 +181,12 @@ def num_threads() -> int:
 
 class OracleSolveUS:
     """The oracle's unstructured triangle solve (orc_run_us) over a mesh from kitbase_amd.mesh; the
-    state (an UnstructState) is modified in place."""
+    state (an UnstructState) is modified in place.  It restates KitBase's interpOrder-1 scheme (the
+    cells' slopes zero), so SolverSets of another order are refused."""
 
     def __init__(self, ks, ps, st, fwall=None, fkind=None):
+        if ks.set.interpOrder != 1:
+            raise ValueError("the oracle's unstructured step restates interpOrder 1 only")
         kb = sys.modules["kitbase_amd"]
         self.lib = load()
         self.keep = []

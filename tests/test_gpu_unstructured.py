@@ -24,9 +24,9 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
-def cfg_of(space, coll="bgk", n=16, kn=0.1):
+def cfg_of(space, coll="bgk", n=16, kn=0.1, order=1):
     mk = kb.configs.cavity_2d2f2v if space == "2d2f2v" else kb.configs.cavity_2d1f2v
-    return mk(nu=n, nv=n, knudsen=kn, collision=coll)
+    return mk(nu=n, nv=n, knudsen=kn, collision=coll, interpOrder=order)
 
 
 def case(space, coll="bgk", n=16, nx=12, ny=10, averaged=()):
@@ -277,6 +277,10 @@ def test_us_refusals():
     ks, ps, st, _ = case("2d1f2v", "shakhov", n=8, nx=3, ny=3)
     with pytest.raises(kb._lib.KbgError, match="Shakhov"):
         kb.mesh.DeviceSolveUS(ks, ps, st)
+    # interpOrder 2 (the second-order unstructured reconstruct!) is not built
+    ks2 = kb.SolverSet(cfg_of("2d2f2v", n=8, order=2))
+    with pytest.raises(kb._lib.KbgError, match="interpOrder"):
+        kb.mesh.DeviceSolveUS(ks2, ps, kb.mesh.init_fvm_us(ks2, ps))
     ks, ps, st, _ = case("2d2f2v", n=8, nx=3, ny=3)
     assert create_raw(ks, ps)[0] == 0
     cf = ps.cellFaces.copy()

@@ -1076,6 +1076,7 @@ def test_cavity_1f2v_symmetry_and_mass():
 # ---------------------------------------------------------------- unstructured triangles (oracle)
 
 def _us_case(cfg, nx=6, ny=5):
+    cfg = dict(cfg, interpOrder=1)  # the unstructured scheme as built: cells' slopes zero
     ks = kb.SolverSet(cfg)
     ps = kb.mesh.tri_rectangle(0.0, 1.0, nx, 0.0, 1.0, ny)
     st = kb.mesh.init_fvm_us(ks, ps)
@@ -1181,7 +1182,7 @@ def test_us_triangle_step_relaxes_like_the_structured_step(coll):
     is the structured 2D2F2V step! (step_2f.jl:179-249, pinned above by the Prandtl-rate test):
     same BGK / Shakhov relaxation of h and b from a distribution that carries heat flux."""
     ks = kb.SolverSet(kb.configs.cavity_2d2f2v(nu=24, nv=24, vMeshType="rectangle", umin=-8.0, umax=8.0,
-                                               vmin=-8.0, vmax=8.0, collision=("bgk", "shakhov")[coll]))
+                                               vmin=-8.0, vmax=8.0, collision=("bgk", "shakhov")[coll], interpOrder=1))
     ps = kb.mesh.tri_rectangle(0.0, 1.0, 1, 0.0, 1.0, 1)
     ps.cellType[:] = 0
     st = kb.mesh.init_fvm_us(ks, ps)
