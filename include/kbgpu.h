@@ -42,8 +42,9 @@ extern "C" {
 enum { KBG_FP32 = 0, KBG_FP64 = 1 };
 /* flux symbols of evolve!(...; mode) -- solver_evolution.jl:55 evaluates "flux_" * mode * "!" */
 enum { KBG_KFVS = 0, KBG_KCU = 1, KBG_UGKS = 2 };
-/* collision symbols of update!(...; coll) -- step_1f.jl:30, step_2f.jl:34 */
-enum { KBG_BGK = 0, KBG_SHAKHOV = 1 };
+/* collision symbols of update!(...; coll) -- step_1f.jl:30, step_2f.jl:34; :fsm is the fast spectral
+ * Boltzmann operator of the 1D1F3V step! (step_1f.jl:122-157, theory_fsm.jl:271-314) */
+enum { KBG_BGK = 0, KBG_SHAKHOV = 1, KBG_FSM = 2 };
 /* boundary symbols: :fix (boundary cells not stepped) and :maxwell (diffuse walls) */
 enum { KBG_BC_FIX = 0, KBG_BC_MAXWELL = 1 };
 /* limiter symbols -- recon_limiter.jl:10-105 */
@@ -75,6 +76,14 @@ typedef struct kbg_desc {
     const double *bc_prim;              /* 1D Maxwell walls: [2][nw] prim at x0 and x1 (KS.ib.bc) */
     const double *xgeo, *ygeo;          /* 2D: per-face [n1, n2, len] (geo_struct.jl normals/areas) */
     const double *xwall, *ywall;        /* 2D: per-face wall prim [4] in the global frame */
+    /* collision KBG_FSM (1D1F3V on a uniform nu x nv x nw grid, u fastest): Gas.fsm of KitBase's
+     * gas_property(vs; ..., collision = "fsm") (struct_property.jl:516-567): Kn = hs_boltz_kn(mu_ref,
+     * alphaRef), nm = M and the kernel modes phi, psi [M(M-1)][nw][nv][nu], phipsi [nw][nv][nu]
+     * (kernel_mode, theory_fsm.jl:24-90).  Ignored for other collisions. */
+    int fsm_nm;
+    int fsm_shape[3];                   /* nu, nv, nw */
+    double fsm_kn;
+    const double *fsm_phi, *fsm_psi, *fsm_phipsi;
 } kbg_desc;
 
 /* Library version (major*10000 + minor*100 + patch) and the calling thread's last error. */

@@ -13,7 +13,7 @@ serving.py (request loop: admission by footprint, continuous batching, p50/p99),
 (schedule tuner), configs.py
 (the workloads), csrc/ (HIP kernels and the C ABI), build.py (hipcc recipe).
 """
-from . import _lib, autotune, configs, geometry, mesh, phase, serving, theory  # noqa: F401
+from . import _lib, autotune, configs, fsm, geometry, mesh, phase, serving, theory  # noqa: F401
 from .solver import (Context, ControlVolumes, DeviceSolve, Interfaces, SolverSet, batch_step, batch_step_cfl,  # noqa: F401
                      batch_timestep, default_context, evolve, init_fvm, make_desc, read_dict, reconstruct, solve,
                      solve_bytes, timestep, update)

@@ -16,14 +16,14 @@ LIB_PATH = Path(__file__).resolve().parent / "lib" / "libkbgpu.so"
 KBG_OK = 0
 KBG_FP32, KBG_FP64 = 0, 1
 KBG_KFVS, KBG_KCU, KBG_UGKS = 0, 1, 2
-KBG_BGK, KBG_SHAKHOV = 0, 1
+KBG_BGK, KBG_SHAKHOV, KBG_FSM = 0, 1, 2
 KBG_BC_FIX, KBG_BC_MAXWELL = 0, 1
 KBG_VANLEER, KBG_MINMOD, KBG_SUPERBEE, KBG_VANALBABA, KBG_LINEAR = 0, 1, 2, 3, 4
 KBG_EV_STEPS_BEGIN, KBG_EV_STEPS_END = 14, 15  # event slots around kbg_step / kbg_batch_step loops
 KBG_HEALTH_NONFINITE, KBG_HEALTH_RHO, KBG_HEALTH_LAMBDA = 1, 2, 4  # kbg_solve_health flag bits
 
 FLUX = {"kfvs": KBG_KFVS, "kcu": KBG_KCU, "ugks": KBG_UGKS}
-COLL = {"bgk": KBG_BGK, "shakhov": KBG_SHAKHOV}
+COLL = {"bgk": KBG_BGK, "shakhov": KBG_SHAKHOV, "fsm": KBG_FSM}
 BC = {"fix": KBG_BC_FIX, "maxwell": KBG_BC_MAXWELL}
 LIMITER = {"vanleer": KBG_VANLEER, "minmod": KBG_MINMOD, "superbee": KBG_SUPERBEE,
            "vanalbaba": KBG_VANALBABA, "linear": KBG_LINEAR}
@@ -44,6 +44,8 @@ class KbgDesc(C.Structure):
         ("u", _dp), ("v", _dp), ("w", _dp), ("weights", _dp),
         ("dx", _dp), ("dy", _dp), ("bc_prim", _dp),
         ("xgeo", _dp), ("ygeo", _dp), ("xwall", _dp), ("ywall", _dp),
+        ("fsm_nm", C.c_int), ("fsm_shape", C.c_int * 3), ("fsm_kn", C.c_double),
+        ("fsm_phi", _dp), ("fsm_psi", _dp), ("fsm_phipsi", _dp),
     ]
 
 

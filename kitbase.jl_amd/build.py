@@ -32,7 +32,10 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     f"-I{ROOT / 'include'}",
 ]
-SOURCES = ["kbg_sweep_f32.hip", "kbg_sweep_f32b.hip", "kbg_sweep_f32c.hip", "kbg_sweep_f64.hip", "kbg_sweep_f64c.hip", "kbg_1d.hip", "kbg_2d.hip", "kbg_step2d.hip", "kbg_us.hip", "kbg_api.hip"]
+# hipFFT for the fast spectral collision's batched 3D transforms (ROCm's own library, found through
+# the image's /opt/rocm)
+LINK = ["-L/opt/rocm/lib", "-lhipfft", "-Wl,-rpath,/opt/rocm/lib"]
+SOURCES = ["kbg_sweep_f32.hip", "kbg_sweep_f32b.hip", "kbg_sweep_f32c.hip", "kbg_sweep_f64.hip", "kbg_sweep_f64c.hip", "kbg_1d.hip", "kbg_2d.hip", "kbg_step2d.hip", "kbg_us.hip", "kbg_fsm.hip", "kbg_api.hip"]
 
 
 def _run(cmds: list[list[str]]) -> None:
@@ -60,7 +63,7 @@ def build_lib() -> Path:
         cmds.append([HIPCC, *HIP_FLAGS, "-c", str(CSRC / src), "-o", str(obj)])
     _run(cmds)
     tmp = LIB.with_suffix(".so.tmp")
-    _run([[HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)]])
+    _run([[HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs), *LINK]])
     os.replace(tmp, LIB)
     return LIB
 

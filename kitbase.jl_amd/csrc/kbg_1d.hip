@@ -314,6 +314,8 @@ template <class T, int NF, int VD> __global__ __launch_bounds__(1024) void k_upd
     const T* WT = (const T*)S.wt;
     const double K = S.K, dt = S.dt, dxc = S.dx[k];
     const bool shak = VD == 1 && S.coll == SHAKHOV && c > 1 && c < nx;
+    // fast spectral step! (step_1f.jl:122-157) for the interior cells; boundary cells take BGK
+    const bool fsm = VD == 3 && S.coll == FSM && c > 1 && c < nx;
     double po[NW], wo[NW];
     for (int m = 0; m < NW; m++) {
         po[m] = prim[m];
@@ -351,7 +353,12 @@ template <class T, int NF, int VD> __global__ __launch_bounds__(1024) void k_upd
         Uo = T(po[1]);
         kbo = T(K / (2.0 * po[NW - 1]));
     }
-    for (int iv = threadIdx.x; iv < nv; iv += blockDim.x) {
+    if (fsm) {
+        const double* __restrict__ Q = S.fsmq + (size_t)(c - S.fsm_c0) * nv;
+        const T dtT = T(dt);
+        for (int iv = threadIdx.x; iv < nv; iv += blockDim.x) f[iv] = f[iv] + (ffL[iv] - ffR[iv]) * rdx + dtT * T(Q[iv]);
+    }
+    for (int iv = threadIdx.x; iv < nv && !fsm; iv += blockDim.x) {
         const T x = U[iv];
         T c2 = (x - Un) * (x - Un);
         if (VD == 3) {

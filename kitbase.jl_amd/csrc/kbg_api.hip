@@ -120,6 +120,9 @@ struct kbg_solve {
     kbg::SolveUS hus{};
     kbg::SolveUS* dus = nullptr;
     int thru = 0;  // threads per workgroup of its face and cell kernels
+    // collision :fsm (1D1F3V): kernel modes, work buffers and hipFFT plans
+    bool has_fsm = false;
+    kbg::FsmPlan fsm{};
     std::vector<void*> allocs;
 };
 
@@ -368,7 +371,16 @@ int validate(const kbg_desc* d) {
     if (!d) return fail(KBG_ERR_ARG, "desc is NULL");
     if (d->precision != KBG_FP32 && d->precision != KBG_FP64) return fail(KBG_ERR_ARG, "precision must be KBG_FP32 or KBG_FP64");
     if (d->flux < KBG_KFVS || d->flux > KBG_UGKS) return fail(KBG_ERR_ARG, "unknown flux");
-    if (d->collision != KBG_BGK && d->collision != KBG_SHAKHOV) return fail(KBG_ERR_ARG, "unknown collision");
+    if (d->collision != KBG_BGK && d->collision != KBG_SHAKHOV && d->collision != KBG_FSM) return fail(KBG_ERR_ARG, "unknown collision");
+    if (d->collision == KBG_FSM) {
+        if (d->dim != 1 || d->vdim != 3 || d->nf != 1)
+            return fail(KBG_ERR_UNSUPPORTED, "the fast spectral collision (:fsm) is KitBase's 1D1F3V step! (step_1f.jl:122-157)");
+        if (d->fsm_nm < 2 || !(d->fsm_kn > 0.0) || !d->fsm_phi || !d->fsm_psi || !d->fsm_phipsi)
+            return fail(KBG_ERR_ARG, ":fsm needs fsm_nm >= 2, fsm_kn > 0 and the kernel modes phi, psi, phipsi");
+        if (d->fsm_shape[0] < 2 || d->fsm_shape[1] < 2 || d->fsm_shape[2] < 2 ||
+            (long)d->fsm_shape[0] * d->fsm_shape[1] * d->fsm_shape[2] != d->nv)
+            return fail(KBG_ERR_ARG, ":fsm needs fsm_shape = (nu, nv, nw) with nu nv nw = nv");
+    }
     if (d->limiter < KBG_VANLEER || d->limiter > KBG_LINEAR) return fail(KBG_ERR_ARG, "unknown limiter");
     if (d->interp_order != 1 && d->interp_order != 2) return fail(KBG_ERR_ARG, "interp_order must be 1 or 2");
     if (d->nv < 1) return fail(KBG_ERR_ARG, "nv must be positive");
@@ -768,8 +780,14 @@ size_t kbg_solve_bytes(const kbg_desc* d) {
         const bool swept = swept_1d(*d);
         const size_t np = swept ? (N + 31) / 32 * 32 : N;
         const size_t vt = swept ? (esz == 8 ? vtab_bytes<double>(d->nv) : vtab_bytes<float>(d->nv)) : 0;
+        size_t fsm = 0;
+        if (d->collision == KBG_FSM && d->nx > 2) {
+            // spectrum, the transforms' products, kernel modes, Q (hipFFT's own work area not counted)
+            const size_t nc = size_t(d->nx) - 2, n3 = size_t(d->nv), nd = size_t(d->fsm_nm) * (d->fsm_nm - 1);
+            fsm = nc * n3 * 16 + nc * (2 * nd + 2) * n3 * 16 + (2 * nd + 1) * n3 * 8 + nc * n3 * 8;
+        }
         return 2 * np * d->nf * d->nv * esz + 3 * N * nw * 8 + N * 8 + 4 * size_t(d->nv) * esz + 2 * d->nf * d->nv * esz +
-               N * 4 * nw * 8 + vt;
+               N * 4 * nw * 8 + vt + fsm;
     }
     const size_t C = size_t(d->nx) * d->ny, F = size_t(d->nx + 1) * d->ny + size_t(d->nx) * (d->ny + 1);
     const size_t nv = d->nv;
@@ -891,6 +909,39 @@ int kbg_solve_create(kbg_ctx* ctx, const kbg_desc* desc, kbg_solve** out) {
             S.vtab = s->vtab;
             if ((rc = ensure_vtab(s, 0.0))) return bail(rc);
         }
+        if (d.collision == KBG_FSM) {
+            // Q for the interior cells 2 .. nx-1 (update! steps them with :fsm; boundary cells take BGK)
+            kbg::FsmPlan& P = s->fsm;
+            P.nu = d.fsm_shape[0];
+            P.nv = d.fsm_shape[1];
+            P.nw = d.fsm_shape[2];
+            P.M = d.fsm_nm;
+            P.kn = d.fsm_kn;
+            P.c0 = 2;
+            P.ncell = d.nx - 2;
+            const size_t n3 = size_t(d.nv), nd = size_t(P.M) * (P.M - 1), nc = size_t(P.ncell);
+            if (S.fvs != 1) return bail(fail(KBG_ERR_STATE, ":fsm needs the velocity-contiguous f layout"));
+            struct {
+                void** dst;
+                size_t bytes;
+            } bufs[] = {{(void**)&P.spec, nc * n3 * 16}, {(void**)&P.work, nc * (2 * nd + 2) * n3 * 16},
+                        {(void**)&P.phi, nd * n3 * 8},   {(void**)&P.psi, nd * n3 * 8},
+                        {(void**)&P.phipsi, n3 * 8},     {(void**)&P.Q, nc * n3 * 8}};
+            for (auto& b : bufs) {
+                if ((rc = dev_alloc(s, &p, b.bytes))) return bail(rc);
+                *b.dst = p;
+            }
+            if ((rc = put_f64(s, P.phi, d.fsm_phi, nd * n3)) || (rc = put_f64(s, P.psi, d.fsm_psi, nd * n3)) ||
+                (rc = put_f64(s, P.phipsi, d.fsm_phipsi, n3)))
+                return bail(rc);
+            HIPCHK(hipMemset(P.Q, 0, nc * n3 * 8));
+            if (kbg::fsm_plan_create(P) != hipSuccess) return bail(fail(KBG_ERR_HIP, "hipfftPlanMany failed for the :fsm transforms"));
+            s->has_fsm = true;
+            S.fsmq = P.Q;
+            S.fsm_c0 = P.c0;
+        }
+        // the descriptor's kernel-mode arrays were copied above; the caller's need not outlive this call
+        s->d.fsm_phi = s->d.fsm_psi = s->d.fsm_phipsi = nullptr;
         if ((rc = push1d(s))) return bail(rc);
     } else {
         const size_t C = size_t(d.nx) * d.ny, FX = size_t(d.nx + 1) * d.ny, FY = size_t(d.nx) * (d.ny + 1);
@@ -1153,6 +1204,7 @@ int kbg_usolve_create(kbg_ctx* ctx, const kbg_udesc* ud, kbg_solve** out) {
 int kbg_solve_destroy(kbg_solve* s) {
     if (!s) return KBG_OK;
     if (s->ctx) (void)hipStreamSynchronize(s->ctx->stream);
+    kbg::fsm_plan_destroy(s->fsm);
     for (void* p : s->allocs) (void)hipFree(p);
     delete s;
     return KBG_OK;
@@ -1667,6 +1719,9 @@ int batch_step_impl(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double*
             if ((rc = pushus(s))) return rc;
             continue;
         }
+        if (noalloc && s->has_fsm)
+            return fail(KBG_ERR_STATE, "kbg_batch_step_noalloc: a solve with the fast spectral collision runs hipFFT, which is "
+                                       "not covered by the allocation-free contract");
         if (noalloc && !(s->d.dim == 1 ? s->h1.sw != nullptr : s->h2.sf != nullptr))
             return fail(KBG_ERR_STATE, "kbg_batch_step_noalloc: a solve stepped by the phase kernels has no slope and face "
                                        "buffers yet (one kbg_step, kbg_batch_step or phase-level call allocates them)");
@@ -1910,7 +1965,10 @@ int batch_step_impl(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double*
     // at both parities of the two-step launches, captured once per launch plan (the signature below: every launch's shape, stream and
     // arguments; descriptors are read from device memory, re-uploaded by every call) and reused
     // by later calls on the same batch.
-    const bool use_graph = ctx->graphs && !residuals && nsteps >= 2 && !noalloc;
+    bool any_fsm = false;
+    for (int i : others) any_fsm = any_fsm || solves[i]->has_fsm;
+    // hipFFT calls stay out of stream capture: batches with :fsm solves launch eagerly
+    const bool use_graph = ctx->graphs && !residuals && nsteps >= 2 && !noalloc && !any_fsm;
     std::string sig;
     if (use_graph) {
         auto put = [&](const void* p, size_t b) { sig.append(static_cast<const char*>(p), b); };
@@ -2047,6 +2105,11 @@ int batch_step_impl(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double*
 int step_phase_1d(kbg_solve* s, double dt, double* residual, hipStream_t st) {
     (void)dt;
     double* rc_ = residual ? s->rcell : nullptr;
+    if (s->has_fsm) {
+        // Q(f^n) of the interior cells before update! changes f (step_1f.jl:146-151)
+        const Solve1D& S = s->h1;
+        HIPCHK(kbg::fsm_collision(s->fsm, s->fp64, S.f[S.cur], S.fcs, s->fsm.c0 - 1 + S.ng, st));
+    }
     HIPCHK(kbg::launch_update1d(s->fp64, s->h1, s->d1, rc_, st));
     if (residual) {
         const int nrows = s->h1.upd_hi - s->h1.upd_lo + 1;

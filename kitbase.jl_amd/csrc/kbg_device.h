@@ -19,7 +19,7 @@ namespace kbg {
 constexpr double kPi = 3.14159265358979323846;
 
 enum Flux { KFVS = 0, KCU = 1, UGKS = 2 };
-enum Coll { BGK = 0, SHAKHOV = 1 };
+enum Coll { BGK = 0, SHAKHOV = 1, FSM = 2 };
 enum Limiter { VANLEER = 0, MINMOD = 1, SUPERBEE = 2, VANALBABA = 3, LINEAR = 4 };
 
 // global-address-space view of a pointer read from a descriptor (so loads and stores are

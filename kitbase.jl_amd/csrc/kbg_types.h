@@ -48,6 +48,8 @@ struct Solve1D {
     const double* bcprim;                          // [2][nw]  wall primitive vectors
     const void* vtab;                              // [nv + 8] per-velocity constants (swept path)
     int* health;                                   // failure flags (kbg_device.h prim_health)
+    const double* fsmq;                            // collision :fsm: Q of the cells fsm_c0 .. [ncell][nv]
+    int fsm_c0;                                    // first physical cell of fsmq
 };
 
 struct Solve2D {
@@ -171,6 +173,26 @@ hipError_t launch_recon1d(bool fp64, const Solve1D& S, const Solve1D* d_S, hipSt
 hipError_t launch_evolve1d(bool fp64, const Solve1D& S, const Solve1D* d_S, hipStream_t st);
 hipError_t launch_wall1d(bool fp64, const Solve1D& S, const Solve1D* d_S, int fused, int par, hipStream_t st);
 hipError_t launch_update1d(bool fp64, const Solve1D& S, const Solve1D* d_S, double* d_rcell, hipStream_t st);
+
+// The fast spectral collision of a 1D1F3V solve (kbg_fsm.hip): kernel modes and work buffers on the
+// device, hipFFT plans for the cells c0 .. c0 + ncell - 1
+struct FsmPlan {
+    int nu = 0, nv = 0, nw = 0, M = 0, ncell = 0, c0 = 0;
+    double kn = 0.0;
+    double2* spec = nullptr;    // [ncell][n3]
+    double2* work = nullptr;    // [ncell][2 M(M-1) + 2][n3]
+    double* phi = nullptr;      // [M(M-1)][n3]
+    double* psi = nullptr;      // [M(M-1)][n3]
+    double* phipsi = nullptr;   // [n3]
+    double* Q = nullptr;        // [ncell][n3]
+    struct hipfftHandle_t* plan_b = nullptr;
+    struct hipfftHandle_t* plan_f = nullptr;
+    bool planned = false;
+};
+// Q of the plan's cells from the f buffer f (cell stride fcs, storage index of cell c0 = k0)
+hipError_t fsm_collision(const FsmPlan& P, bool fp64, const void* f, long fcs, int k0, hipStream_t st);
+hipError_t fsm_plan_create(FsmPlan& P);
+void fsm_plan_destroy(FsmPlan& P);
 hipError_t launch_resid1d(const Solve1D& S, const double* d_rcell, int nrows, int boundary_split, double* d_res,
                           hipStream_t st);
 

@@ -6,7 +6,8 @@ Julia glue for libkbgpu.so: device-resident KitBase solves stepped on an MI355X 
 `reconstruct!`, `evolve!`, `update!`, `timestep` and `solve!`; `download!` writes the state back into
 the host `ControlVolume` / `Interface` arrays (for `extract_sol`, `write_jld`, plotting).
 
-Supported: 1D1F1V, 1D2F1V and 1D1F3V (`ControlVolume1F/2F` over `eachindex(ps.x)`, ghosts included;
+Supported: 1D1F1V, 1D2F1V and 1D1F3V (also with KitBase's fast spectral collision, `collision =
+"fsm"`, its `Gas.fsm` kernel modes passed through) (`ControlVolume1F/2F` over `eachindex(ps.x)`, ghosts included;
 `Interface1F/2F` faces 1..nx+1), 2D1F2V and 2D2F2V without ghost cells (`ControlVolume1F/2F`
 matrices with `a1face` (nx+1) x ny and `a2face` nx x (ny+1)); fluxes :kfvs, :kcu, :ugks; collisions
 :bgk, :shakhov (not 1D1F3V, whose KitBase step! cannot run it, step_1f.jl:91, nor one-distribution
@@ -48,6 +49,8 @@ struct KbgDesc
     u::Ptr{Cdouble}; v::Ptr{Cdouble}; w::Ptr{Cdouble}; weights::Ptr{Cdouble}
     dx::Ptr{Cdouble}; dy::Ptr{Cdouble}; bc_prim::Ptr{Cdouble}
     xgeo::Ptr{Cdouble}; ygeo::Ptr{Cdouble}; xwall::Ptr{Cdouble}; ywall::Ptr{Cdouble}
+    fsm_nm::Cint; fsm_shape::NTuple{3,Cint}; fsm_kn::Cdouble
+    fsm_phi::Ptr{Cdouble}; fsm_psi::Ptr{Cdouble}; fsm_phipsi::Ptr{Cdouble}
 end
 
 # kbg_udesc (include/kbgpu.h), field for field
@@ -62,7 +65,7 @@ struct KbgUDesc
 end
 
 const FLUX = Dict(:kfvs => 0, :kcu => 1, :ugks => 2)
-const COLL = Dict(:bgk => 0, :shakhov => 1)
+const COLL = Dict(:bgk => 0, :shakhov => 1, :fsm => 2)
 const BCS = Dict(:fix => 0, :maxwell => 1)
 const LIMITER = Dict(:vanleer => 0, :minmod => 1, :superbee => 2, :vanalbaba => 3, :linear => 4)
 
@@ -178,12 +181,19 @@ function make_desc(ks::SolverSet, precision, keep)
     prec = precision == :fp64 ? Cint(1) : Cint(0)
     common = (lookup(FLUX, ks.set.flux, "flux"), lookup(COLL, ks.set.collision, "collision"),
               lookup(LIMITER, ks.set.limiter, "limiter"), ks.set.interpOrder, boundary_codes(ks.set))
+    # collision :fsm: Gas.fsm = (Kn, nm, ϕ, ψ, χ) of gas_property(vs; ..., collision = "fsm")
+    # (struct_property.jl:516-567), kernel modes [u, v, w, idx] column-major as the library reads them
+    fsm = if Symbol(ks.set.collision) == :fsm
+        (Cint(gas.fsm.nm), (Cint(vs.nu), Cint(vs.nv), Cint(vs.nw)), gas.fsm.Kn, ptr(gas.fsm.ϕ), ptr(gas.fsm.ψ), ptr(gas.fsm.χ))
+    else
+        (Cint(0), (Cint(0), Cint(0), Cint(0)), 0.0, nul, nul, nul)
+    end
     if dim == 1
         ng = 1 - first(axes(ps.x, 1))                  # ghost layers of the OffsetArray, update_boundary.jl:40
         dx = ptr(ps.dx)                                # parent: cells 1-ng .. nx+ng
         bcp = ptr(vcat(ks.ib.bc(ps.x0, ks.ib.p), ks.ib.bc(ps.x1, ks.ib.p)))
         return KbgDesc(1, nf, vdim, length(vs.u), ps.nx, 1, ng, prec, common...,
-                       gas.K, gas.γ, gas.μᵣ, gas.ω, gas.Pr, u, v, ww, w, dx, nul, bcp, nul, nul, nul, nul)
+                       gas.K, gas.γ, gas.μᵣ, gas.ω, gas.Pr, u, v, ww, w, dx, nul, bcp, nul, nul, nul, nul, fsm...)
     elseif dim == 2
         (first(axes(ps.x, 1)) == 1 && first(axes(ps.x, 2)) == 1) ||
             error("libkbgpu steps 2D solves without ghost cells (PSpace2D with NGX = NGY = 0)")
@@ -191,7 +201,7 @@ function make_desc(ks::SolverSet, precision, keep)
         xw, yw = wall_prims_2d(ks)
         return KbgDesc(2, nf, vdim, length(vs.u), ps.nx, ps.ny, 0, prec, common...,
                        gas.K, gas.γ, gas.μᵣ, gas.ω, gas.Pr, u, v, ww, w, ptr(ps.dx), ptr(ps.dy), nul,
-                       ptr(xg), ptr(yg), ptr(xw), ptr(yw))
+                       ptr(xg), ptr(yg), ptr(xw), ptr(yw), fsm...)
     end
     error("space $(space) is not supported by libkbgpu")
 end

@@ -144,6 +144,12 @@ class VSpace3D:
     v: np.ndarray
     w: np.ndarray
     weights: np.ndarray
+    v0: float = 0.0
+    v1: float = 0.0
+    w0: float = 0.0
+    w1: float = 0.0
+    shape: tuple = (0, 0, 0)        # (nu, nv, nw): KitBase's per-axis counts
+    spacing: tuple = (0.0, 0.0, 0.0)  # (du, dv, dw) of a uniform grid (du[1, 1, 1], ...)
 
     @property
     def nv(self) -> int:
@@ -167,4 +173,4 @@ def vspace3d(U0, U1, NU, V0, V1, NV, W0, W1, NW, type: str = "rectangle") -> VSp
         Wt = (np.tile(cu, NV * NW) * du) * np.tile(np.repeat(cv, NU), NW) * dv * np.repeat(cw, NU * NV) * dw
     else:
         raise ValueError("No velocity quadrature available")
-    return VSpace3D(U0, U1, NU, U, V, Wv, Wt)
+    return VSpace3D(U0, U1, NU, U, V, Wv, Wt, V0, V1, W0, W1, (NU, NV, NW), (du, dv, dw))

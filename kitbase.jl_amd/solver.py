@@ -90,6 +90,7 @@ class Gas:
     alphaRef: float = 1.0
     omegaRef: float = 0.5
     mu_ref: float = 0.0
+    fsm: object = None  # collision "fsm": fsm.FsmKernel (Kn, nm, phi, psi, phipsi)
 
 
 def gas_property(space: str, inK: float, knudsen: float, mach: float, prandtl: float, omega: float = 0.5,
@@ -251,6 +252,12 @@ class SolverSet:
                                cfg["wmin"], cfg["wmax"], cfg["nw"], mt)
         self.gas = gas_property(sp, cfg["inK"], cfg["knudsen"], cfg.get("mach", 0.0), cfg.get("prandtl", 1.0),
                                 cfg.get("omega", 0.5), cfg.get("alphaRef", 1.0), cfg.get("omegaRef", 0.5))
+        if self.set.collision == "fsm":
+            # gas_property(vs; ..., collision = "fsm", nm) (struct_property.jl:516-567)
+            if sp != "1d1f3v" or mt != "rectangle":
+                raise NotImplementedError("the fast spectral collision is built for 1D1F3V on a rectangle velocity grid")
+            from .fsm import fsm_kernel
+            self.gas.fsm = fsm_kernel(self.vs, self.gas.mu_ref, int(cfg.get("nm", 5)), self.gas.alphaRef)
         case = self.set.case
         if case in ("shock", "rh"):
             self.ib = ib_rh(self.set, self.ps, self.vs, self.gas)
@@ -509,6 +516,11 @@ def make_desc(ks: SolverSet, precision: str = "fp64") -> tuple[_lib.KbgDesc, lis
         d.v = arr(vs.v)
     if ks.vdim == 3:
         d.w = arr(vs.w)
+    if ks.set.collision == "fsm":
+        fk = ks.gas.fsm
+        d.fsm_nm, d.fsm_kn = fk.nm, fk.Kn
+        d.fsm_shape[0], d.fsm_shape[1], d.fsm_shape[2] = vs.shape
+        d.fsm_phi, d.fsm_psi, d.fsm_phipsi = arr(fk.phi), arr(fk.psi), arr(fk.phipsi)
     if ks.dim == 1:
         ps = ks.ps
         d.nx, d.ny, d.ng = ps.nx, 1, ps.ng

@@ -14,6 +14,7 @@
  * to rounding (summation order of `sum` is Julia's pairwise/SIMD order in KitBase and sequential
  * here, a difference of a few ulp).
  */
+#define _DEFAULT_SOURCE /* j1 (Bessel J1) from libm, for the fast-spectral kernel modes */
 #include "kitbase_oracle.h"
 
 #include <math.h>
@@ -1387,6 +1388,9 @@ static int step_cell_1d(orc_solver1d *S, int i, double dt, double *RES, double *
         orc_step_2f1v(w, prim, f, f + nv, fwL, ffL, ffL + nv, fwR, ffR, ffR + nv, S->u, S->weights, nv, S->K, S->gam, S->mu, S->om, S->Pr, S->dx[k], dt, RES, AVG, coll);
         return 0;
     }
+    if (coll == ORC_FSM)
+        return orc_step_1f3v_fsm(w, prim, f, fwL, ffL, fwR, ffR, S->gam, S->fsm_kn, S->fsm_nm, S->fsm_phi, S->fsm_psi,
+                                 S->fsm_phipsi, S->fsm_nu, S->fsm_nv, S->fsm_nw, S->dx[k], dt, RES, AVG);
     return orc_step_1f3v(w, prim, f, fwL, ffL, fwR, ffR, S->u, S->v, S->wv, S->weights, nv, S->gam, S->mu, S->om, S->Pr, S->dx[k], dt, RES, AVG, coll);
 }
 
@@ -1864,5 +1868,218 @@ int orc_run_us(orc_solverus *S, double dt, int nsteps, double *res) {
         if (!rc) rc = orc_update_us(S, dt, res);
         if (rc) return rc;
     }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Fast spectral method (1D1F3V collision :fsm)                                                 */
+/* ------------------------------------------------------------------------------------------ */
+
+/* lgwt   src/Math/math_general.jl:58-98: N Gauss-Legendre nodes and weights on [a, b] by the damped
+ * Newton iteration of KitBase (its derivative carries (N+1) for N, undone in the weights); nodes in
+ * descending order. */
+void orc_lgwt(int N, double a, double b, double *x, double *w) {
+    const int N1 = N, N2 = N + 1;
+    double *y = (double *)malloc(sizeof(double) * N1 * 3), *y0 = y + N1, *Lp = y0 + N1;
+    double *L = (double *)malloc(sizeof(double) * N1 * N2);
+    for (int i = 1; i <= N1; i++) {
+        y[i - 1] = cos((2.0 * (i - 1.0) + 1.0) * 4.0 * atan(1.0) / (2.0 * (N - 1.0) + 2.0)) +
+                   0.27 / N1 * sin(4.0 * atan(1.0) * (-1.0 + i * 2.0 / (N1 - 1.0)) * (N - 1.0) / N2);
+        y0[i - 1] = 2.0;
+    }
+    for (;;) {
+        double dmax = 0.0;
+        for (int i = 0; i < N1; i++) dmax = fmax(dmax, fabs(y[i] - y0[i]));
+        if (!(dmax > 0.0000000000001)) break;
+        for (int i = 0; i < N1; i++) {
+            double *Li = L + (size_t)i * N2;  /* L[i, 1..N2] */
+            Li[0] = 1.0;
+            Li[1] = y[i];
+            for (int k = 2; k <= N1; k++) Li[k] = ((2.0 * k - 1.0) * y[i] * Li[k - 1] - (k - 1) * Li[k - 2]) / k;
+            Lp[i] = N2 * (Li[N1 - 1] - y[i] * Li[N2 - 1]) / (1.0 - y[i] * y[i]);
+            y0[i] = y[i];
+            y[i] = y0[i] - Li[N2 - 1] / Lp[i];
+        }
+    }
+    for (int i = 0; i < N1; i++) {
+        x[i] = (a * (1.0 - y[i]) + b * (1.0 + y[i])) / 2.0;
+        w[i] = (double)N2 * N2 * (b - a) / ((1.0 - y[i] * y[i]) * Lp[i] * Lp[i]) / ((double)N1 * N1);
+    }
+    free(L);
+    free(y);
+}
+
+/* hs_boltz_kn   src/Theory/theory_fsm.jl:16-17 (Gamma(2) = 1) */
+double orc_hs_boltz_kn(double mu_ref, double alpha) {
+    return 64.0 * pow(sqrt(2.0), alpha) / 5.0 * tgamma((alpha + 3.0) / 2.0) * tgamma(2.0) * sqrt(PI_) * mu_ref;
+}
+
+/* kernel_mode   src/Theory/theory_fsm.jl:24-90 (the variant fsm_kernel calls, with the grid
+ * spacings): phi, psi [M(M-1)][nw][nv][nu] and phipsi [nw][nv][nu], u fastest (Julia's column-major
+ * phi[i, j, k, idx]).  besselj(1, .) is libm's j1. */
+void orc_kernel_mode(int M, double umax, double vmax, double wmax, double du, double dv, double dw, int nu, int nv,
+                     int nw, double alpha, int quad_num, double *phi, double *psi, double *phipsi) {
+    const double supp = sqrt(2.0) * 2.0 * fmax(fmax(umax, vmax), wmax) / (3.0 + sqrt(2.0));
+    double *fx = (double *)malloc(sizeof(double) * (nu + nv + nw)), *fy = fx + nu, *fz = fy + nv;
+    /* range(-pi/d, (n/2 - 1) 2 pi / n / d; length = n): start + k step */
+    for (int i = 0; i < nu; i++) fx[i] = -PI_ / du + i * (((nu / 2 - 1) * 2.0 * PI_ / nu / du - (-PI_ / du)) / (nu - 1));
+    for (int j = 0; j < nv; j++) fy[j] = -PI_ / dv + j * (((nv / 2 - 1) * 2.0 * PI_ / nv / dv - (-PI_ / dv)) / (nv - 1));
+    for (int k = 0; k < nw; k++) fz[k] = -PI_ / dw + k * (((nw / 2 - 1) * 2.0 * PI_ / nw / dw - (-PI_ / dw)) / (nw - 1));
+    double *ab = (double *)malloc(sizeof(double) * quad_num * 2), *gw = ab + quad_num;
+    orc_lgwt(quad_num, 0.0, supp, ab, gw);
+    const size_t n3 = (size_t)nu * nv * nw;
+    for (size_t q = 0; q < n3; q++) phipsi[q] = 0.0;
+    for (int loop = 1; loop <= M - 1; loop++) {
+        const double theta = PI_ / M * loop;
+        for (int loop2 = 1; loop2 <= M; loop2++) {
+            const double theta2 = PI_ / M * loop2;
+            const int idx = (loop - 1) * M + loop2 - 1;
+#pragma omp parallel for schedule(static)
+            for (int k = 0; k < nw; k++)
+                for (int j = 0; j < nv; j++)
+                    for (int i = 0; i < nu; i++) {
+                        const size_t q = (size_t)i + (size_t)nu * (j + (size_t)nv * k);
+                        double s = fx[i] * sin(theta) * cos(theta2) + fy[j] * sin(theta) * sin(theta2) + fz[k] * cos(theta);
+                        double int_temp = 0.0;
+                        for (int id = 0; id < quad_num; id++) int_temp += 2.0 * gw[id] * cos(s * ab[id]) * pow(ab[id], alpha);
+                        const double ph = int_temp * sin(theta);
+                        s = fx[i] * fx[i] + fy[j] * fy[j] + fz[k] * fz[k] - s * s;
+                        double ps;
+                        if (s <= 0.0) {
+                            ps = PI_ * supp * supp;
+                        } else {
+                            s = sqrt(s);
+                            ps = 2.0 * PI_ * supp * j1(supp * s) / s;
+                        }
+                        phi[(size_t)idx * n3 + q] = ph;
+                        psi[(size_t)idx * n3 + q] = ps;
+                        phipsi[q] += ph * ps;
+                    }
+        }
+    }
+    free(ab);
+    free(fx);
+}
+
+/* one axis of an unnormalised 3D DFT over [nw][nv][nu] (u fastest): sign -1 forward (fft), +1
+ * backward (bfft), X_k = sum_n x_n exp(sign 2 pi i k n / N) */
+static void dft_axis(double *re, double *im, int nu, int nv, int nw, int axis, int sign) {
+    const int n = axis == 0 ? nu : (axis == 1 ? nv : nw);
+    const size_t stride = axis == 0 ? 1 : (axis == 1 ? (size_t)nu : (size_t)nu * nv);
+    const size_t lines = (size_t)nu * nv * nw / n;
+    double *cs = (double *)malloc(sizeof(double) * n * 2), *sn = cs + n;
+    for (int t = 0; t < n; t++) {
+        cs[t] = cos(2.0 * PI_ * t / n);
+        sn[t] = sign * sin(2.0 * PI_ * t / n);
+    }
+#pragma omp parallel
+    {
+        double *xr = (double *)malloc(sizeof(double) * n * 2), *xi = xr + n;
+#pragma omp for schedule(static)
+        for (size_t l = 0; l < lines; l++) {
+            /* the line's base: every index with the axis coordinate zero */
+            size_t base;
+            if (axis == 0) base = l * nu;
+            else if (axis == 1) base = (l % nu) + (l / nu) * (size_t)nu * nv;
+            else base = l;
+            for (int t = 0; t < n; t++) {
+                xr[t] = re[base + t * stride];
+                xi[t] = im[base + t * stride];
+            }
+            for (int kk = 0; kk < n; kk++) {
+                double ar = 0.0, ai = 0.0;
+                for (int t = 0; t < n; t++) {
+                    const int e = (int)(((long)kk * t) % n);
+                    ar += xr[t] * cs[e] - xi[t] * sn[e];
+                    ai += xr[t] * sn[e] + xi[t] * cs[e];
+                }
+                re[base + kk * stride] = ar;
+                im[base + kk * stride] = ai;
+            }
+        }
+        free(xr);
+    }
+    free(cs);
+}
+
+void orc_dft3(double *re, double *im, int nu, int nv, int nw, int sign) {
+    for (int a = 0; a < 3; a++) dft_axis(re, im, nu, nv, nw, a, sign);
+}
+
+/* fftshift of a [nw][nv][nu] array (each dimension rolled by n / 2) */
+void orc_fftshift3(const double *in, double *out, int nu, int nv, int nw) {
+    for (int k = 0; k < nw; k++)
+        for (int j = 0; j < nv; j++)
+            for (int i = 0; i < nu; i++) {
+                const int i2 = (i + nu / 2) % nu, j2 = (j + nv / 2) % nv, k2 = (k + nw / 2) % nw;
+                out[(size_t)i2 + (size_t)nu * (j2 + (size_t)nv * k2)] = in[(size_t)i + (size_t)nu * (j + (size_t)nv * k)];
+            }
+}
+
+/* boltzmann_fft!   src/Theory/theory_fsm.jl:271-314: f_spec = fftshift(bfft(f) / N^3); gain
+ * sum_i fft(f_spec phi_i) fft(f_spec psi_i), loss fft(f_spec phipsi) fft(f_spec);
+ * Q = 4 pi^2 / Kn / M^2 Re(gain - loss) */
+void orc_boltzmann_fft(double *Q, const double *f, double Kn, int M, const double *phi, const double *psi,
+                       const double *phipsi, int nu, int nv, int nw) {
+    const size_t n3 = (size_t)nu * nv * nw;
+    double *sr = (double *)calloc(n3 * 8, sizeof(double));
+    double *si = sr + n3, *ar = si + n3, *ai = ar + n3, *br = ai + n3, *bi = br + n3, *tr = bi + n3, *ti = tr + n3;
+    memcpy(ar, f, n3 * sizeof(double));
+    orc_dft3(ar, ai, nu, nv, nw, +1);
+    const double inv = 1.0 / (double)n3;
+    for (size_t q = 0; q < n3; q++) {
+        ar[q] *= inv;
+        ai[q] *= inv;
+    }
+    orc_fftshift3(ar, sr, nu, nv, nw);
+    orc_fftshift3(ai, si, nu, nv, nw);
+    for (size_t q = 0; q < n3; q++) tr[q] = ti[q] = 0.0;
+    for (int d = 0; d < M * (M - 1); d++) {
+        const double *ph = phi + (size_t)d * n3, *ps = psi + (size_t)d * n3;
+        for (size_t q = 0; q < n3; q++) {
+            ar[q] = sr[q] * ph[q];
+            ai[q] = si[q] * ph[q];
+            br[q] = sr[q] * ps[q];
+            bi[q] = si[q] * ps[q];
+        }
+        orc_dft3(ar, ai, nu, nv, nw, -1);
+        orc_dft3(br, bi, nu, nv, nw, -1);
+        for (size_t q = 0; q < n3; q++) {
+            tr[q] += ar[q] * br[q] - ai[q] * bi[q];
+            ti[q] += ar[q] * bi[q] + ai[q] * br[q];
+        }
+    }
+    for (size_t q = 0; q < n3; q++) {
+        ar[q] = sr[q] * phipsi[q];
+        ai[q] = si[q] * phipsi[q];
+        br[q] = sr[q];
+        bi[q] = si[q];
+    }
+    orc_dft3(ar, ai, nu, nv, nw, -1);
+    orc_dft3(br, bi, nu, nv, nw, -1);
+    for (size_t q = 0; q < n3; q++) tr[q] -= ar[q] * br[q] - ai[q] * bi[q];
+    const double c = 4.0 * PI_ * PI_ / Kn / ((double)M * M);
+    for (size_t q = 0; q < n3; q++) Q[q] = c * tr[q];
+    free(sr);
+}
+
+/* 1D1F3V fast spectral step!   src/Solver/step_1f.jl:122-157: w += (fwL - fwR) / dx, prim, the
+ * residual sums, then f += (ffL - ffR) / dx + dt Q(f) with Q from f before the update */
+int orc_step_1f3v_fsm(double *w, double *prim, double *f, const double *fwL, const double *ffL, const double *fwR,
+                      const double *ffR, double gam, double Kn, int M, const double *phi, const double *psi,
+                      const double *phipsi, int nu, int nv, int nw, double dx, double dt, double *RES, double *AVG) {
+    double w_old[5];
+    memcpy(w_old, w, sizeof w_old);
+    for (int m = 0; m < 5; m++) w[m] += (fwL[m] - fwR[m]) / dx;
+    orc_conserve_prim(w, 5, gam, prim);
+    for (int m = 0; m < 5; m++) {
+        RES[m] += (w[m] - w_old[m]) * (w[m] - w_old[m]);
+        AVG[m] += fabs(w[m]);
+    }
+    const size_t n3 = (size_t)nu * nv * nw;
+    double *Q = (double *)malloc(sizeof(double) * n3);
+    orc_boltzmann_fft(Q, f, Kn, M, phi, psi, phipsi, nu, nv, nw);
+    for (size_t i = 0; i < n3; i++) f[i] += (ffL[i] - ffR[i]) / dx + dt * Q[i];
+    free(Q);
     return 0;
 }

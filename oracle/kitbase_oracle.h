@@ -19,7 +19,7 @@ extern "C" {
 
 /* enum values shared with the product ABI (include/kbgpu.h) */
 enum { ORC_KFVS = 0, ORC_KCU = 1, ORC_UGKS = 2 };
-enum { ORC_BGK = 0, ORC_SHAKHOV = 1 };
+enum { ORC_BGK = 0, ORC_SHAKHOV = 1, ORC_FSM = 2 };
 enum { ORC_BC_FIX = 0, ORC_BC_MAXWELL = 1 };
 enum { ORC_VANLEER = 0, ORC_MINMOD = 1, ORC_SUPERBEE = 2, ORC_VANALBABA = 3, ORC_LINEAR = 4 };
 enum { ORC_RECTANGLE = 0, ORC_NEWTON = 1, ORC_ALGEBRA = 2 };
@@ -152,6 +152,10 @@ typedef struct {
     const double *bc_prim[2];       /* wall primitive vectors (nw values) for Maxwell walls */
     double *w, *prim, *f, *sw, *sf; /* cell state */
     double *fw, *ff;                /* face state */
+    /* collision :fsm (1D1F3V): Gas.fsm = (Kn, nm, phi, psi, phipsi) and the velocity grid's shape */
+    int fsm_nm, fsm_nu, fsm_nv, fsm_nw;
+    double fsm_kn;
+    const double *fsm_phi, *fsm_psi, *fsm_phipsi;
 } orc_solver1d;
 
 int orc_reconstruct_1d(orc_solver1d *S);                    /* solver_reconstruction.jl:56-153 */
@@ -220,6 +224,20 @@ typedef struct {
 int orc_evolve_us(orc_solverus *S, double dt);
 int orc_update_us(orc_solverus *S, double dt, double *res);
 int orc_run_us(orc_solverus *S, double dt, int nsteps, double *res);
+
+/* ---- fast spectral method (1D1F3V :fsm) ---- */
+void orc_lgwt(int N, double a, double b, double *x, double *w);             /* math_general.jl:58-98 */
+double orc_hs_boltz_kn(double mu_ref, double alpha);                        /* theory_fsm.jl:16-17 */
+void orc_kernel_mode(int M, double umax, double vmax, double wmax, double du, double dv, double dw, int nu, int nv,
+                     int nw, double alpha, int quad_num, double *phi, double *psi, double *phipsi); /* theory_fsm.jl:24-90 */
+void orc_dft3(double *re, double *im, int nu, int nv, int nw, int sign);    /* sign -1 fft, +1 bfft (unnormalised) */
+void orc_fftshift3(const double *in, double *out, int nu, int nv, int nw);
+void orc_boltzmann_fft(double *Q, const double *f, double Kn, int M, const double *phi, const double *psi,
+                       const double *phipsi, int nu, int nv, int nw);       /* theory_fsm.jl:271-314 */
+int orc_step_1f3v_fsm(double *w, double *prim, double *f, const double *fwL, const double *ffL, const double *fwR,
+                      const double *ffR, double gam, double Kn, int M, const double *phi, const double *psi,
+                      const double *phipsi, int nu, int nv, int nw, double dx, double dt, double *RES,
+                      double *AVG);                                          /* step_1f.jl:122-157 */
 
 int orc_num_threads(void);
 void orc_set_num_threads(int n);

@@ -65,6 +65,8 @@ class Solver1DStruct(C.Structure):
         ("K", C.c_double), ("gam", C.c_double), ("mu", C.c_double), ("om", C.c_double), ("Pr", C.c_double),
         ("u", _dp), ("v", _dp), ("wv", _dp), ("weights", _dp), ("dx", _dp), ("bc_prim", _dp * 2),
         ("w", _dp), ("prim", _dp), ("f", _dp), ("sw", _dp), ("sf", _dp), ("fw", _dp), ("ff", _dp),
+        ("fsm_nm", C.c_int), ("fsm_nu", C.c_int), ("fsm_nv", C.c_int), ("fsm_nw", C.c_int),
+        ("fsm_kn", C.c_double), ("fsm_phi", _dp), ("fsm_psi", _dp), ("fsm_phipsi", _dp),
     ]
 
 
@@ -95,7 +97,7 @@ class SolverUSStruct(C.Structure):
 
 
 FLUX = {"kfvs": 0, "kcu": 1, "ugks": 2}
-COLL = {"bgk": 0, "shakhov": 1}
+COLL = {"bgk": 0, "shakhov": 1, "fsm": 2}
 BC = {"fix": 0, "maxwell": 1}
 LIM = {"vanleer": 0, "minmod": 1, "superbee": 2, "vanalbaba": 3, "linear": 4}
 
@@ -129,6 +131,10 @@ class OracleSolve:
             S.bc_prim[0], S.bc_prim[1] = k(ks.ib.bc(ps.x0)), k(ks.ib.bc(ps.x1))
             S.w, S.prim, S.f, S.sw, S.sf = p(ctr.w), p(ctr.prim), p(ctr.f), p(ctr.sw), p(ctr.sf)
             S.fw, S.ff = p(face.fw), p(face.ff)
+            if ks.set.collision == "fsm":
+                fk = ks.gas.fsm
+                S.fsm_nm, (S.fsm_nu, S.fsm_nv, S.fsm_nw) = fk.nm, vs.shape
+                S.fsm_kn, S.fsm_phi, S.fsm_psi, S.fsm_phipsi = fk.Kn, k(fk.phi), k(fk.psi), k(fk.phipsi)
         else:
             kb = sys.modules["kitbase_amd"]  # the host mirror supplies the face geometry
             S = Solver2DStruct()

@@ -1208,3 +1208,146 @@ def test_us_triangle_step_relaxes_like_the_structured_step(coll):
         np.testing.assert_allclose(st.f[c, 0], h, rtol=1e-12, atol=1e-300)
         np.testing.assert_allclose(st.f[c, 1], b, rtol=1e-12, atol=1e-300)
         np.testing.assert_allclose(st.w[c], W2, rtol=1e-13)
+
+
+# ---------------------------------------------------------------- fast spectral method (1D1F3V :fsm)
+
+def _fsm_setup(n=16, umax=8.0, mu=0.05, alpha=1.0, nm=5):
+    vs = kb.phase.vspace3d(-umax, umax, n, -umax, umax, n, -umax, umax, n)
+    fk = kb.fsm.fsm_kernel(vs, mu, nm, alpha)
+    return vs, fk
+
+
+def _fsm_q(vs, fk, f):
+    n = vs.shape
+    q = A(np.zeros(vs.nv))
+    call("orc_boltzmann_fft", q, A(f), fk.Kn, fk.nm, A(fk.phi), A(fk.psi), A(fk.phipsi), n[0], n[1], n[2])
+    return q
+
+
+def _maxw3(vs, rho, U, lam):
+    return rho * (lam / np.pi) ** 1.5 * np.exp(-lam * ((vs.u - U[0]) ** 2 + (vs.v - U[1]) ** 2 + (vs.w - U[2]) ** 2))
+
+
+def test_fsm_lgwt_is_gauss_legendre():
+    """lgwt (math_general.jl:58-98): the Gauss-Legendre rule on [a, b] -- numpy's nodes and weights,
+    and exact for polynomials up to degree 2N - 1.  KitBase stops the Newton iteration at a step of
+    1e-13 and takes the weights from the derivative at the previous iterate, so they carry a
+    relative error of about N^2 1e-13."""
+    for N, b in ((5, 1.0), (64, 6.4)):
+        x, w = A(np.zeros(N)), A(np.zeros(N))
+        call("orc_lgwt", N, 0.0, b, x, w)
+        xl, wl = np.polynomial.legendre.leggauss(N)
+        o = np.argsort(x)
+        np.testing.assert_allclose(x[o], (xl + 1) * b / 2, rtol=0, atol=1e-13 * b)
+        np.testing.assert_allclose(w[o], wl * b / 2, rtol=1e-9)
+        for k in range(2 * N):
+            assert np.dot(w, x ** k) == pytest.approx(b ** (k + 1) / (k + 1), rel=1e-9)
+
+
+def test_fsm_dft_and_fftshift_follow_fftw_conventions():
+    """The oracle's transforms are numpy's (FFTW's conventions, which KitBase's FFTW calls use): sign
+    -1 is fftn, sign +1 is the unnormalised inverse (bfft = n^3 ifftn), and fftshift rolls every axis
+    by n // 2 -- on a ragged [nw][nv][nu] = [5][6][4] array."""
+    rng = np.random.default_rng(3)
+    nu, nv, nw = 4, 6, 5
+    x = rng.standard_normal((nw, nv, nu)) + 1j * rng.standard_normal((nw, nv, nu))
+    for sign, ref in ((-1, np.fft.fftn(x)), (1, np.fft.ifftn(x) * x.size)):
+        re, im = A(x.real.ravel()), A(x.imag.ravel())
+        call("orc_dft3", re, im, nu, nv, nw, sign)
+        np.testing.assert_allclose(re + 1j * im, ref.ravel(), rtol=0, atol=1e-12 * np.abs(ref).max())
+    out = A(np.zeros(x.size))
+    call("orc_fftshift3", A(x.real.ravel()), out, nu, nv, nw)
+    np.testing.assert_array_equal(out, np.fft.fftshift(x.real).ravel())
+
+
+def test_fsm_kernel_modes_closed_forms():
+    """kernel_mode (theory_fsm.jl:24-90) for hard spheres (alpha = 1): phi is sin(theta) times
+    2 int_0^R x cos(s x) dx = 2 ((cos(sR) - 1) / s^2 + R sin(sR) / s) (2 R^2 / 2 at s = 0); psi at a
+    frequency parallel to the direction is pi R^2, the r -> 0 limit of 2 pi R J1(R r) / r; phipsi
+    is the sum of phi psi over the directions; and the oracle's modes equal the host set-up's."""
+    vs, fk = _fsm_setup(n=8)
+    n, M = 8, fk.nm
+    R = math.sqrt(2.0) * 2.0 * 8.0 / (3.0 + math.sqrt(2.0))
+    phi, psi, pp = (A(np.zeros((M * (M - 1), n, n, n))), A(np.zeros((M * (M - 1), n, n, n))), A(np.zeros((n, n, n))))
+    du = 16.0 / n
+    call("orc_kernel_mode", M, 8.0, 8.0, 8.0, du, du, du, n, n, n, 1.0, 64, phi, psi, pp)
+    fr = -np.pi / du + np.arange(n) * (2 * np.pi / (n * du))
+    for loop in range(1, M):
+        th = np.pi / M * loop
+        for loop2 in range(1, M + 1):
+            th2 = np.pi / M * loop2
+            d = (loop - 1) * M + loop2 - 1
+            s = (fr[None, None, :] * math.sin(th) * math.cos(th2) + fr[None, :, None] * math.sin(th) * math.sin(th2)
+                 + fr[:, None, None] * math.cos(th))
+            with np.errstate(invalid="ignore", divide="ignore"):
+                exact = np.where(np.abs(s) < 1e-12, R ** 2,
+                                 2 * ((np.cos(s * R) - 1) / s ** 2 + R * np.sin(s * R) / s)) * math.sin(th)
+            np.testing.assert_allclose(phi[d], exact, rtol=0, atol=1e-10 * R ** 2)
+    # zero frequency (index n/2 on every axis): parallel to every direction
+    np.testing.assert_allclose(psi[:, n // 2, n // 2, n // 2], np.pi * R ** 2, rtol=1e-14)
+    small = 2 * np.pi * R * special.j1(R * 1e-6) / 1e-6
+    assert small == pytest.approx(np.pi * R ** 2, rel=1e-9)
+    np.testing.assert_allclose(pp, np.einsum("dijk,dijk->ijk", phi, psi), rtol=1e-13, atol=1e-13 * np.abs(pp).max())
+    np.testing.assert_allclose(phi, fk.phi, rtol=1e-12, atol=1e-12 * np.abs(phi).max())
+    np.testing.assert_allclose(psi, fk.psi, rtol=1e-12, atol=1e-12 * np.abs(psi).max())
+
+
+def test_fsm_collision_vanishes_at_equilibrium_and_conserves():
+    """boltzmann_fft! (theory_fsm.jl:271-314): Q of a Maxwellian is zero up to the spectral error
+    (gain and loss cancel: a wrong factor on either, or a dropped fftshift, breaks it), and for a
+    far-from-equilibrium f the collision conserves mass and momentum to rounding and energy to the
+    method's own accuracy."""
+    vs, fk = _fsm_setup()
+    q0 = _fsm_q(vs, fk, _maxw3(vs, 1.0, (0.3, -0.2, 0.1), 0.3))  # resolved on the 16^3 grid of spacing 1
+    f = 0.5 * _maxw3(vs, 1.0, (1.2, 0, 0), 0.3) + 0.5 * _maxw3(vs, 1.0, (-1.2, 0, 0), 0.3)
+    q = _fsm_q(vs, fk, f)
+    scale = np.abs(q).max()
+    assert scale > 1e-3
+    assert np.abs(q0).max() < 5e-4 * scale
+    wt, mag = vs.weights, float(np.sum(np.abs(q) * vs.weights))
+    assert abs(np.sum(q * wt)) < 1e-12 * mag
+    for c in (vs.u, vs.v, vs.w):
+        assert abs(np.sum(q * wt * c)) < 1e-12 * mag
+    # the spectral method does not conserve energy exactly; on this coarse grid, a few 1e-3
+    assert abs(np.sum(q * wt * (vs.u ** 2 + vs.v ** 2 + vs.w ** 2))) < 1e-2 * mag
+
+
+def test_fsm_relaxes_a_bimodal_toward_isotropy():
+    """Homogeneous relaxation f += dt Q: the bimodal along u carries T_xx > T_yy; collisions move
+    energy between the directions, so the anisotropy decays monotonically while mass stays put."""
+    vs, fk = _fsm_setup()
+    f = 0.5 * _maxw3(vs, 1.0, (1.2, 0, 0), 0.6) + 0.5 * _maxw3(vs, 1.0, (-1.2, 0, 0), 0.6)
+    wt = vs.weights
+    m0 = np.sum(f * wt)
+    aniso = []
+    for _ in range(6):
+        rho = np.sum(f * wt)
+        aniso.append((np.sum(f * wt * vs.u ** 2) - np.sum(f * wt * vs.v ** 2)) / rho)
+        f = f + 0.05 * _fsm_q(vs, fk, f)
+    assert all(b < a for a, b in zip(aniso, aniso[1:])), aniso
+    assert aniso[-1] < 0.9 * aniso[0]
+    assert np.sum(f * wt) == pytest.approx(m0, rel=1e-12)
+
+
+def test_fsm_step_is_transport_plus_dt_q():
+    """step!(:fsm) (step_1f.jl:122-157): with no face fluxes w and prim stay and f moves by dt Q(f),
+    Q taken from f before the update."""
+    vs, fk = _fsm_setup(n=8)
+    f0 = 0.5 * _maxw3(vs, 1.0, (1.0, 0, 0), 0.6) + 0.5 * _maxw3(vs, 1.0, (-1.0, 0, 0), 0.6)
+    wt = vs.weights
+    w = A([np.sum(f0 * wt), np.sum(f0 * wt * vs.u), np.sum(f0 * wt * vs.v), np.sum(f0 * wt * vs.w),
+           0.5 * np.sum(f0 * wt * (vs.u ** 2 + vs.v ** 2 + vs.w ** 2))])
+    gam = 5.0 / 3.0
+    prim = A(np.zeros(5))
+    call("orc_conserve_prim", w, 5, gam, prim)
+    q = _fsm_q(vs, fk, f0)
+    f = A(f0.copy())
+    z5, zn = A(np.zeros(5)), A(np.zeros(vs.nv))
+    res, avg = A(np.zeros(5)), A(np.zeros(5))
+    w1, p1 = A(w.copy()), A(np.zeros(5))
+    call("orc_step_1f3v_fsm", w1, p1, f, z5, zn, z5, zn, gam, fk.Kn, fk.nm, A(fk.phi), A(fk.psi), A(fk.phipsi),
+         8, 8, 8, 0.5, 0.01, res, avg)
+    np.testing.assert_array_equal(w1, w)
+    np.testing.assert_allclose(p1, prim, rtol=1e-15)
+    np.testing.assert_allclose(f, f0 + 0.01 * q, rtol=1e-14, atol=1e-16)
