@@ -208,6 +208,27 @@ def unstructured_leg(kb, ctx, steps, nx=64, nu=28):
     return out
 
 
+def fsm_leg(kb, ctx, steps):
+    """1D1F3V shock structure with KitBase's fast spectral collision (50 cells, 24^3 velocities,
+    M = 5: 20 direction pairs) stepped on the device; device time from the library's step events."""
+    ks = kb.SolverSet(kb.configs.shock_1d1f3v(collision="fsm"))
+    out = {"config": f"1D1F3V shock, {ks.ps.nx} cells x 24^3 velocities, KFVS + fast spectral collision (M = 5)",
+           "steps": steps, "transforms_per_step": (ks.ps.nx - 2) * (2 * 20 + 2) + (ks.ps.nx - 2)}
+    for prec in ("fp64", "fp32"):
+        ctr, face = kb.init_fvm(ks)
+        dt = kb.timestep(ks, ctr)
+        d = kb.DeviceSolve(ks, ctr, face, precision=prec, ctx=ctx)
+        d.step(dt, 2)
+        ctx.sync()
+        t1 = time.perf_counter()
+        d.step(dt, steps)
+        wall_c = time.perf_counter() - t1
+        out[prec] = {"ms_per_step": wall_c * 1e3 / steps, "device_ms_per_step": ctx.last_steps_ms() / steps}
+        d.close()
+    log(f"[bench] fsm: fp64 {out['fp64']['device_ms_per_step']:.3f} ms/step, fp32 {out['fp32']['device_ms_per_step']:.3f} ms/step")
+    return out
+
+
 def serving_leg(kb, ctx, n, rates, chunk, overlap):
     out = {"mix": "75% 1D tubes of the batch (to t = 0.2), 15% lid-driven cavities {32,48,64,96,128}^2 cells x "
                   "{16,24,32,48}^2 velocities (100 steps), 10% 1D1F3V 24 cells x 24^3 velocities (40 steps); FP32",
@@ -242,6 +263,8 @@ def main():
     ap.add_argument("--cavity-only", action="store_true", help="run only the 2D cavity leg (profiling)")
     ap.add_argument("--us-steps", type=int, default=50, help="timed steps of the unstructured cavity leg (0 skips it)")
     ap.add_argument("--us-only", action="store_true", help="run only the unstructured cavity leg (profiling)")
+    ap.add_argument("--fsm-steps", type=int, default=10, help="timed steps of the fast spectral 1D1F3V leg (0 skips it)")
+    ap.add_argument("--fsm-only", action="store_true", help="run only the fast spectral leg (profiling)")
     ap.add_argument("--serve", type=int, default=48, metavar="N",
                     help="serving leg: N requests of the mixed stream per arrival rate (0 skips it)")
     ap.add_argument("--serve-rates", default="10,20,50,100,200,500", help="Poisson arrival rates of the serving leg (1/s)")
@@ -282,6 +305,9 @@ def main():
         return
     if a.us_only:
         print(json.dumps({"unstructured_2d": unstructured_leg(kb, kb.Context(0), max(1, a.us_steps))}), flush=True)
+        return
+    if a.fsm_only:
+        print(json.dumps({"fsm_1d1f3v": fsm_leg(kb, kb.Context(0), max(1, a.fsm_steps))}), flush=True)
         return
     K, W = a.steps, a.warmup
     t0 = time.time()
@@ -375,6 +401,8 @@ def main():
     cavity = cavity_leg(kb, ctx, a.cavity_steps) if a.cavity_steps else None
     # ---- the same cavity on a triangulated mesh (8192 triangles x 28^2 velocities), fused unstructured step
     unstructured = unstructured_leg(kb, ctx, a.us_steps) if a.us_steps else None
+    # ---- 1D1F3V with the fast spectral Boltzmann collision (50 cells x 24^3 velocities)
+    fsm = fsm_leg(kb, ctx, a.fsm_steps) if a.fsm_steps else None
 
     # ---- serving leg: Poisson arrivals of 1D tubes, 2D cavities and 1D1F3V requests, served together
     serve = None
@@ -450,6 +478,7 @@ def main():
         "schedule": tuned,
         "cavity_2d": cavity,
         "unstructured_2d": unstructured,
+        "fsm_1d1f3v": fsm,
         "fp64_plain": fp64,
         "speedup_vs_fp64_plain": (value / fp64["value"]) if fp64 else None,
         "l2_vs_fp64": acc,
