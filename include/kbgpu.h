@@ -79,7 +79,8 @@ typedef struct kbg_desc {
     /* collision KBG_FSM (1D1F3V on a uniform nu x nv x nw grid, u fastest): Gas.fsm of KitBase's
      * gas_property(vs; ..., collision = "fsm") (struct_property.jl:516-567): Kn = hs_boltz_kn(mu_ref,
      * alphaRef), nm = M and the kernel modes phi, psi [M(M-1)][nw][nv][nu], phipsi [nw][nv][nu]
-     * (kernel_mode, theory_fsm.jl:24-90).  Ignored for other collisions. */
+     * (kernel_mode, theory_fsm.jl:24-90).  The operator's transforms run in the storage precision
+     * (Q is summed in FP64).  Ignored for other collisions. */
     int fsm_nm;
     int fsm_shape[3];                   /* nu, nv, nw */
     double fsm_kn;

@@ -169,6 +169,15 @@ int put_f64(kbg_solve* s, double* dst, const double* src, size_t n) {
     return KBG_OK;
 }
 
+// FP64 host values to device values of the solve's real type (FP32 when f32)
+int put_real(kbg_solve* s, void* dst, const double* src, size_t n, bool f32) {
+    if (!f32) return put_f64(s, (double*)dst, src, n);
+    std::vector<float> tmp(src, src + n);
+    HIPCHK(hipMemcpyAsync(dst, tmp.data(), n * 4, hipMemcpyHostToDevice, s->ctx->stream));
+    HIPCHK(hipStreamSynchronize(s->ctx->stream));
+    return KBG_OK;
+}
+
 int get_f64(kbg_solve* s, double* dst, const double* src, size_t n) {
     HIPCHK(hipMemcpyAsync(dst, src, n * 8, hipMemcpyDeviceToHost, s->ctx->stream));
     HIPCHK(hipStreamSynchronize(s->ctx->stream));
@@ -782,9 +791,12 @@ size_t kbg_solve_bytes(const kbg_desc* d) {
         const size_t vt = swept ? (esz == 8 ? vtab_bytes<double>(d->nv) : vtab_bytes<float>(d->nv)) : 0;
         size_t fsm = 0;
         if (d->collision == KBG_FSM && d->nx > 2) {
-            // spectrum, the transforms' products, kernel modes, Q (hipFFT's own work area not counted)
+            // spectrum, the transforms' products, kernel modes (all in the storage precision), Q (FP64);
+            // hipFFT's own work area not counted
             const size_t nc = size_t(d->nx) - 2, n3 = size_t(d->nv), nd = size_t(d->fsm_nm) * (d->fsm_nm - 1);
-            fsm = nc * n3 * 16 + nc * (2 * nd + 2) * n3 * 16 + (2 * nd + 1) * n3 * 8 + nc * n3 * 8;
+            const int pn = kbg::fsm_pair_size(esz == 4, d->fsm_shape[0], d->fsm_shape[1], d->fsm_shape[2]);
+            const size_t work = pn ? nc * (nd + 1) * n3 * 4 + size_t(pn) * 8 : nc * (2 * nd + 2) * n3 * 2 * esz;
+            fsm = nc * n3 * 2 * esz + work + (2 * nd + 1) * n3 * esz + nc * n3 * 8;
         }
         return 2 * np * d->nf * d->nv * esz + 3 * N * nw * 8 + N * 8 + 4 * size_t(d->nv) * esz + 2 * d->nf * d->nv * esz +
                N * 4 * nw * 8 + vt + fsm;
@@ -919,21 +931,35 @@ int kbg_solve_create(kbg_ctx* ctx, const kbg_desc* desc, kbg_solve** out) {
             P.kn = d.fsm_kn;
             P.c0 = 2;
             P.ncell = d.nx - 2;
-            const size_t n3 = size_t(d.nv), nd = size_t(P.M) * (P.M - 1), nc = size_t(P.ncell);
+            P.f32 = !s->fp64;
+            P.pair_n = kbg::fsm_pair_size(P.f32, P.nu, P.nv, P.nw);
+            const size_t n3 = size_t(d.nv), nd = size_t(P.M) * (P.M - 1), nc = size_t(P.ncell), re = P.f32 ? 4 : 8;
             if (S.fvs != 1) return bail(fail(KBG_ERR_STATE, ":fsm needs the velocity-contiguous f layout"));
+            // the LDS path keeps one FP32 product per pair instead of the transformed mode products
+            const size_t work = P.pair_n ? 0 : nc * (2 * nd + 2) * n3 * 2 * re;
+            const size_t part = P.pair_n ? nc * (nd + 1) * n3 * 4 : 0, twb = P.pair_n ? size_t(P.pair_n) * 8 : 0;
             struct {
                 void** dst;
                 size_t bytes;
-            } bufs[] = {{(void**)&P.spec, nc * n3 * 16}, {(void**)&P.work, nc * (2 * nd + 2) * n3 * 16},
-                        {(void**)&P.phi, nd * n3 * 8},   {(void**)&P.psi, nd * n3 * 8},
-                        {(void**)&P.phipsi, n3 * 8},     {(void**)&P.Q, nc * n3 * 8}};
+            } bufs[] = {{&P.spec, nc * n3 * 2 * re}, {&P.work, work},       {&P.phi, nd * n3 * re},
+                        {&P.psi, nd * n3 * re},      {&P.phipsi, n3 * re},  {(void**)&P.Q, nc * n3 * 8},
+                        {(void**)&P.part, part},     {&P.tw, twb}};
             for (auto& b : bufs) {
+                if (!b.bytes) continue;
                 if ((rc = dev_alloc(s, &p, b.bytes))) return bail(rc);
                 *b.dst = p;
             }
-            if ((rc = put_f64(s, P.phi, d.fsm_phi, nd * n3)) || (rc = put_f64(s, P.psi, d.fsm_psi, nd * n3)) ||
-                (rc = put_f64(s, P.phipsi, d.fsm_phipsi, n3)))
+            if ((rc = put_real(s, P.phi, d.fsm_phi, nd * n3, P.f32)) || (rc = put_real(s, P.psi, d.fsm_psi, nd * n3, P.f32)) ||
+                (rc = put_real(s, P.phipsi, d.fsm_phipsi, n3, P.f32)))
                 return bail(rc);
+            if (P.pair_n) {
+                std::vector<double> tw(2 * P.pair_n);
+                for (int j = 0; j < P.pair_n; j++) {
+                    tw[2 * j] = std::cos(-2.0 * kbg::kPi * j / P.pair_n);
+                    tw[2 * j + 1] = std::sin(-2.0 * kbg::kPi * j / P.pair_n);
+                }
+                if ((rc = put_real(s, P.tw, tw.data(), tw.size(), true))) return bail(rc);
+            }
             HIPCHK(hipMemset(P.Q, 0, nc * n3 * 8));
             if (kbg::fsm_plan_create(P) != hipSuccess) return bail(fail(KBG_ERR_HIP, "hipfftPlanMany failed for the :fsm transforms"));
             s->has_fsm = true;

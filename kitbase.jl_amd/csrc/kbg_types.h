@@ -179,12 +179,18 @@ hipError_t launch_update1d(bool fp64, const Solve1D& S, const Solve1D* d_S, doub
 struct FsmPlan {
     int nu = 0, nv = 0, nw = 0, M = 0, ncell = 0, c0 = 0;
     double kn = 0.0;
-    double2* spec = nullptr;    // [ncell][n3]
-    double2* work = nullptr;    // [ncell][2 M(M-1) + 2][n3]
-    double* phi = nullptr;      // [M(M-1)][n3]
-    double* psi = nullptr;      // [M(M-1)][n3]
-    double* phipsi = nullptr;   // [n3]
+    bool f32 = false;           // transforms, spectra and modes in FP32 (FP32 storage), else FP64
+    void* spec = nullptr;       // complex [ncell][n3]
+    void* work = nullptr;       // complex [ncell][2 M(M-1) + 2][n3]
+    void* phi = nullptr;        // real [M(M-1)][n3]
+    void* psi = nullptr;        // real [M(M-1)][n3]
+    void* phipsi = nullptr;     // real [n3]
     double* Q = nullptr;        // [ncell][n3]
+    // FP32 cubes of side pair_n (fsm_pair_size, else 0): forward transforms in LDS, per-pair
+    // products part [ncell][M(M-1) + 1][n3], twiddles tw[pair_n] = exp(-2 pi i j / pair_n); no work
+    int pair_n = 0;
+    float* part = nullptr;
+    void* tw = nullptr;
     struct hipfftHandle_t* plan_b = nullptr;
     struct hipfftHandle_t* plan_f = nullptr;
     bool planned = false;
@@ -192,6 +198,8 @@ struct FsmPlan {
 // Q of the plan's cells from the f buffer f (cell stride fcs, storage index of cell c0 = k0)
 hipError_t fsm_collision(const FsmPlan& P, bool fp64, const void* f, long fcs, int k0, hipStream_t st);
 hipError_t fsm_plan_create(FsmPlan& P);
+// side of the cubic velocity grids whose FP32 forward transforms run in LDS (8, 12, 16, 18, 24), else 0
+int fsm_pair_size(bool f32, int nu, int nv, int nw);
 void fsm_plan_destroy(FsmPlan& P);
 hipError_t launch_resid1d(const Solve1D& S, const double* d_rcell, int nrows, int boundary_split, double* d_res,
                           hipStream_t st);

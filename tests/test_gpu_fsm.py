@@ -2,7 +2,8 @@
 3D transforms plus the mode products) against the oracle's orc_step_1f3v_fsm, through the C ABI.
 
 Tolerances: FP64 storage <= 1e-10 relative (max norm of f and w after a few steps); FP32 storage
-(f in FP32, the collision operator in FP64) <= 1e-4 relative L2 in density, velocity and temperature.
+(f and the transforms in FP32, Q summed in FP64) <= 1e-4 relative L2 in density, velocity and
+temperature, and in the dt Q increment of a homogeneous gas.
 """
 import copy
 import ctypes as C
@@ -25,8 +26,9 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
-def fsm_case(n=16, nx=8, kn=1.0):
-    cfg = kb.configs.shock_1d1f3v(collision="fsm", nu=n, nv=n, nw=n, umin=-8.0, umax=8.0, vmin=-8.0, vmax=8.0,
+def fsm_case(n=16, nx=8, kn=1.0, shape=None):
+    nu, nv, nw = shape or (n, n, n)
+    cfg = kb.configs.shock_1d1f3v(collision="fsm", nu=nu, nv=nv, nw=nw, umin=-8.0, umax=8.0, vmin=-8.0, vmax=8.0,
                                   wmin=-8.0, wmax=8.0, nx=nx, knudsen=kn)
     ks = kb.SolverSet(cfg)
     ctr, face = kb.init_fvm(ks)
@@ -62,6 +64,46 @@ def test_fsm_fp32_matches_the_oracle():
     assert rel_l2(pg[:, 0], po[:, 0]) <= 1e-4
     assert rel_l2(1 / pg[:, 4], 1 / po[:, 4]) <= 1e-4
     assert float(np.linalg.norm(pg[:, 1:4] - po[:, 1:4]) / np.linalg.norm(po[:, 1:4])) <= 1e-4
+
+
+def homogeneous_bimodal(ks, ctr):
+    """Every cell holds the same bimodal f (two Maxwellians at u = +-1.2): the face fluxes cancel, so
+    a step moves f by exactly dt Q(f) (step_1f.jl:146-151) and w stays."""
+    vs = ks.vs
+
+    def maxw(U, lam):
+        return (lam / np.pi) ** 1.5 * np.exp(-lam * ((vs.u - U) ** 2 + vs.v ** 2 + vs.w ** 2))
+
+    fb = 0.5 * maxw(1.2, 0.6) + 0.5 * maxw(-1.2, 0.6)
+    wt = vs.weights
+    w = np.array([np.sum(fb * wt), np.sum(fb * vs.u * wt), np.sum(fb * vs.v * wt), np.sum(fb * vs.w * wt),
+                  0.5 * np.sum(fb * (vs.u ** 2 + vs.v ** 2 + vs.w ** 2) * wt)])
+    rho = w[0]
+    lam = 0.75 * rho / (w[4] - 0.5 * np.sum(w[1:4] ** 2) / rho)  # monatomic, K = 0
+    ctr.f[:, 0, :] = fb
+    ctr.w[:] = w
+    ctr.prim[:] = [rho, w[1] / rho, w[2] / rho, w[3] / rho, lam]
+
+
+@pytest.mark.parametrize("precision,tol,shape", [
+    ("fp64", 1e-10, (16, 16, 16)), ("fp64", 1e-10, (16, 12, 12)),
+    # FP32 cubes of side 8, 12, 16, 18, 24 transform in LDS (one kernel instance per side); other
+    # shapes through hipFFT in FP32
+    ("fp32", 1e-4, (8, 8, 8)), ("fp32", 1e-4, (12, 12, 12)), ("fp32", 1e-4, (16, 16, 16)),
+    ("fp32", 1e-4, (18, 18, 18)), ("fp32", 1e-4, (24, 24, 24)), ("fp32", 1e-4, (16, 12, 12))])
+def test_fsm_homogeneous_step_is_dt_q(precision, tol, shape):
+    """The collision operator alone: in a homogeneous gas one step moves f by dt Q(f) (64 % of f on
+    16^3).  FP64 storage runs the transforms in FP64 (bar 1e-10), FP32 storage in FP32 (bar 1e-4
+    relative L2 of the increment; the spectra, modes and transforms round at 6e-8)."""
+    ks, ctr, face, dt = fsm_case(nx=6, shape=shape)
+    homogeneous_bimodal(ks, ctr)
+    cg, co = run_both(ks, ctr, face, dt, 1, precision)
+    dg, do = cg.f[1:-1] - ctr.f[1:-1], co.f[1:-1] - ctr.f[1:-1]
+    assert np.linalg.norm(do) > 0.02 * np.linalg.norm(ctr.f[1:-1])
+    err = rel_l2(dg, do)
+    print(f"{precision}: relative L2 of the dt Q increment {err:.3e}")
+    assert err <= tol, err
+    np.testing.assert_allclose(cg.w, co.w, rtol=1e-12, atol=0)
 
 
 def test_fsm_phase_calls_match_the_oracle():
