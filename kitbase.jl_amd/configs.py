@@ -106,12 +106,13 @@ SERVE_SEED = 20260318
 
 def serving_requests(n: int = 48, rate: float = 100.0, seed: int = SERVE_SEED, mix=(0.75, 0.15, 0.10),
                      cavity_cells=(32, 48, 64, 96, 128), cavity_vel=(16, 24, 32, 48), v3_cells: int = 24,
-                     cavity_steps: int = 100, v3_steps: int = 40):
+                     cavity_steps: int = 100, v3_steps: int = 40, v3_collision: str = "fsm"):
     """The serving stream: Poisson arrivals at `rate` requests/s mixing
       * 1D shock tubes of the batched workload (run to t_end = 0.2, KitBase's nt = floor(T/dt) + 1),
       * lid-driven cavities (2D2F2V UGKS) at cavity_cells^2 cells x cavity_vel^2 velocities
         (Maxwell quadrature up to 32 per axis, a rectangle grid above),
-      * 1D1F3V shock structures (24^3 velocities, Rankine-Hugoniot states, BGK).
+      * 1D1F3V shock structures (24^3 velocities, Rankine-Hugoniot states, the fast spectral
+        Boltzmann collision by default, v3_collision = "bgk" for BGK).
     Returns (kind, cfg, nsteps or None, arrival) tuples; nsteps None means "to maxTime"."""
     rng = np.random.default_rng(seed)
     t = 0.0
@@ -128,5 +129,5 @@ def serving_requests(n: int = 48, rate: float = 100.0, seed: int = SERVE_SEED, m
             out.append(("2d", cavity_2d2f2v(nx=nc, ny=nc, nu=nvel, nv=nvel,
                                             vMeshType="maxwell" if nvel <= 32 else "rectangle"), cavity_steps, t))
         else:
-            out.append(("3v", shock_1d1f3v(nx=v3_cells, nu=24, nv=24, nw=24), v3_steps, t))
+            out.append(("3v", shock_1d1f3v(nx=v3_cells, nu=24, nv=24, nw=24, collision=v3_collision), v3_steps, t))
     return out

@@ -28,6 +28,10 @@ def _mix():
                                    nsteps=17, arrival=0.004))
     reqs.append(kb.serving.Request(rid=12, cfg=kb.configs.shock_1d1f3v(nx=14, nu=6, nv=6, nw=6), nsteps=9, arrival=0.003))
     reqs.append(kb.serving.Request(rid=13, cfg=kb.configs.shock_1d2f1v(nx=50, nu=30), nsteps=31, arrival=0.0))
+    # the fast spectral collision, as the bench mix's 24^3 requests take it (8^3: the LDS transforms)
+    reqs.append(kb.serving.Request(rid=14, cfg=kb.configs.shock_1d1f3v(nx=10, nu=8, nv=8, nw=8, umin=-8.0, umax=8.0, vmin=-8.0,
+                                                                       vmax=8.0, wmin=-8.0, wmax=8.0, collision="fsm"),
+                                   nsteps=11, arrival=0.0025))
     return reqs
 
 
