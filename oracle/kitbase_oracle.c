@@ -9,6 +9,11 @@
  * does.  OpenMP threads the cell and face loops the way KitBase threads them with @threads, so the
  * same code doubles as the CPU baseline.
  *
+ * Parity unpinned (no golden vector, closed form or brute-force check fixes them, only their
+ * limits and consistency): the UGKS time coefficients Mt at intermediate tau (pinned at tau -> 0 and
+ * tau -> infinity and against the discrete moments), and multi-step trajectories with all of
+ * KitBase's quirks combined.
+ *
  * Citations are relative to the KitBase.jl source tree.  Julia evaluates broadcast expressions
  * left to right; the C below keeps the same operation order so the FP64 results agree with KitBase
  * to rounding (summation order of `sum` is Julia's pairwise/SIMD order in KitBase and sequential

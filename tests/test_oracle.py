@@ -1330,6 +1330,36 @@ def test_fsm_relaxes_a_bimodal_toward_isotropy():
     assert np.sum(f * wt) == pytest.approx(m0, rel=1e-12)
 
 
+def _stress_rate_times_tau(n, umax, kn=1.0, lam=1.0, nm=5):
+    """Initial decay rate of P_xx - P_yy under Q for f = M (1 + eps lam (u^2 - v^2)), times the BGK
+    collision time tau = mu / p of the same hard-sphere gas (vhs_collision_time with omega = 1/2)."""
+    alpha, omega, eps = 1.0, 0.5, 1e-3
+    mu = kb.theory.ref_vhs_vis(kn, alpha, omega)
+    vs = kb.phase.vspace3d(-umax, umax, n, -umax, umax, n, -umax, umax, n)
+    fk = kb.fsm.fsm_kernel(vs, mu, nm, alpha)
+    f = _maxw3(vs, 1.0, (0.0, 0.0, 0.0), lam) * (1.0 + eps * lam * (vs.u ** 2 - vs.v ** 2))
+    q = _fsm_q(vs, fk, f)
+    an = vs.weights * (vs.u ** 2 - vs.v ** 2)
+    rate = -np.sum(an * q) / np.sum(an * f)
+    return rate * kb.theory.vhs_collision_time(1.0, lam, mu, omega)
+
+
+def test_fsm_stress_relaxes_at_the_hard_sphere_viscosity_rate():
+    """The absolute rate of Q: 4 pi^2 / (Kn M^2) (theory_fsm.jl:271-314) with Kn = hs_boltz_kn
+    (theory_fsm.jl:16-17).  For a small anisotropy phi = u^2 - v^2 of a Maxwellian, the initial decay
+    rate of P_xx - P_yy is the Rayleigh quotient <phi, L phi> / <phi, phi> of the linearised
+    hard-sphere operator, which is p / mu_1 with mu_1 the first Chapman-Enskog viscosity -- the
+    viscosity KitBase's mu_ref stands for, so the rate is 1 / tau of its BGK relaxation
+    (vhs_collision_time, omega = 1/2).  The discretised operator meets it within 2 % (truncation and
+    quadrature) on 16^3 to 24^3 grids, at two temperatures and with 5 or 6 directions; a factor of 2
+    in the prefactor, M for M^2 or a wrong hs_boltz_kn fails.  The rate scales exactly as 1 / Kn."""
+    for n, umax, lam, nm in ((16, 6.0, 1.0, 5), (24, 8.0, 1.0, 5), (24, 8.0, 0.5, 5), (24, 8.0, 1.0, 6)):
+        r = _stress_rate_times_tau(n, umax, lam=lam, nm=nm)
+        assert abs(r - 1.0) < 0.02, (n, umax, lam, nm, r)
+    r1, r01 = _stress_rate_times_tau(16, 6.0, kn=1.0), _stress_rate_times_tau(16, 6.0, kn=0.1)
+    assert abs(r01 / r1 - 1.0) < 1e-12
+
+
 def test_fsm_step_is_transport_plus_dt_q():
     """step!(:fsm) (step_1f.jl:122-157): with no face fluxes w and prim stay and f moves by dt Q(f),
     Q taken from f before the update."""
