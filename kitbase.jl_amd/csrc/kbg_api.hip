@@ -794,8 +794,8 @@ size_t kbg_solve_bytes(const kbg_desc* d) {
             // spectrum, the transforms' products, kernel modes (all in the storage precision), Q (FP64);
             // hipFFT's own work area not counted
             const size_t nc = size_t(d->nx) - 2, n3 = size_t(d->nv), nd = size_t(d->fsm_nm) * (d->fsm_nm - 1);
-            const int pn = kbg::fsm_pair_size(esz == 4, d->fsm_shape[0], d->fsm_shape[1], d->fsm_shape[2]);
-            const size_t work = pn ? nc * (nd + 1) * n3 * 4 + size_t(pn) * 8 : nc * (2 * nd + 2) * n3 * 2 * esz;
+            const int cn = kbg::fsm_cube_size(d->fsm_shape[0], d->fsm_shape[1], d->fsm_shape[2]);
+            const size_t work = (cn && esz == 4 ? nc * (nd + 1) * n3 * 4 : nc * (2 * nd + 2) * n3 * 2 * esz) + size_t(cn) * 2 * esz;
             fsm = nc * n3 * 2 * esz + work + (2 * nd + 1) * n3 * esz + nc * n3 * 8;
         }
         return 2 * np * d->nf * d->nv * esz + 3 * N * nw * 8 + N * 8 + 4 * size_t(d->nv) * esz + 2 * d->nf * d->nv * esz +
@@ -932,12 +932,13 @@ int kbg_solve_create(kbg_ctx* ctx, const kbg_desc* desc, kbg_solve** out) {
             P.c0 = 2;
             P.ncell = d.nx - 2;
             P.f32 = !s->fp64;
-            P.pair_n = kbg::fsm_pair_size(P.f32, P.nu, P.nv, P.nw);
+            P.cube_n = kbg::fsm_cube_size(P.nu, P.nv, P.nw);
             const size_t n3 = size_t(d.nv), nd = size_t(P.M) * (P.M - 1), nc = size_t(P.ncell), re = P.f32 ? 4 : 8;
             if (S.fvs != 1) return bail(fail(KBG_ERR_STATE, ":fsm needs the velocity-contiguous f layout"));
-            // the LDS path keeps one FP32 product per pair instead of the transformed mode products
-            const size_t work = P.pair_n ? 0 : nc * (2 * nd + 2) * n3 * 2 * re;
-            const size_t part = P.pair_n ? nc * (nd + 1) * n3 * 4 : 0, twb = P.pair_n ? size_t(P.pair_n) * 8 : 0;
+            // the FP32 LDS path keeps one FP32 product per pair instead of the transformed mode products
+            const bool lds = P.cube_n && P.f32;
+            const size_t work = lds ? 0 : nc * (2 * nd + 2) * n3 * 2 * re;
+            const size_t part = lds ? nc * (nd + 1) * n3 * 4 : 0, twb = size_t(P.cube_n) * 2 * re;
             struct {
                 void** dst;
                 size_t bytes;
@@ -952,13 +953,13 @@ int kbg_solve_create(kbg_ctx* ctx, const kbg_desc* desc, kbg_solve** out) {
             if ((rc = put_real(s, P.phi, d.fsm_phi, nd * n3, P.f32)) || (rc = put_real(s, P.psi, d.fsm_psi, nd * n3, P.f32)) ||
                 (rc = put_real(s, P.phipsi, d.fsm_phipsi, n3, P.f32)))
                 return bail(rc);
-            if (P.pair_n) {
-                std::vector<double> tw(2 * P.pair_n);
-                for (int j = 0; j < P.pair_n; j++) {
-                    tw[2 * j] = std::cos(-2.0 * kbg::kPi * j / P.pair_n);
-                    tw[2 * j + 1] = std::sin(-2.0 * kbg::kPi * j / P.pair_n);
+            if (P.cube_n) {
+                std::vector<double> tw(2 * P.cube_n);
+                for (int j = 0; j < P.cube_n; j++) {
+                    tw[2 * j] = std::cos(-2.0 * kbg::kPi * j / P.cube_n);
+                    tw[2 * j + 1] = std::sin(-2.0 * kbg::kPi * j / P.cube_n);
                 }
-                if ((rc = put_real(s, P.tw, tw.data(), tw.size(), true))) return bail(rc);
+                if ((rc = put_real(s, P.tw, tw.data(), tw.size(), P.f32))) return bail(rc);
             }
             HIPCHK(hipMemset(P.Q, 0, nc * n3 * 8));
             if (kbg::fsm_plan_create(P) != hipSuccess) return bail(fail(KBG_ERR_HIP, "hipfftPlanMany failed for the :fsm transforms"));

@@ -10,8 +10,10 @@
 //   k_fsm_q      Q = 4 pi^2 / Kn / M^2 Re(sum_d F[f_spec phi_d] F[f_spec psi_d]
 //                                      - F[f_spec phipsi] F[f_spec])   (sums in FP64)
 // The transforms, the spectra and the kernel modes are FP64 for FP64 storage and FP32 for FP32
-// storage (R below): the step streams the 2 M(M-1) + 2 product arrays per cell through HBM about a
-// dozen times, so the FP32 transforms halve its bytes.  Q is FP64 in both.  The kernel modes phi,
+// storage (R below).  hipFFT's forward pass streams the 2 M(M-1) + 2 product arrays per cell through
+// HBM about a dozen times; on cubic grids of side 8, 12, 16, 18 or 24 the forward transforms are the
+// library's own pencil transforms instead (kbg_fft.h): FP32 entirely in LDS (k_fsm_pair), FP64
+// written once and read once (k_fsm_slab64, k_fsm_q).  Q is FP64 in all of them.  The kernel modes phi,
 // psi, phipsi (kernel_mode, theory_fsm.jl:24-90) come from the host with the descriptor.  Arrays are
 // [w][v][u] with u fastest (KitBase's column-major f[i, j, k]).  The update kernel then adds dt Q to
 // the transport update (k_update1d).
@@ -96,7 +98,7 @@ __global__ __launch_bounds__(256) void k_fsm_q(const cx_t<R>* __restrict__ work,
     Q[i] = scale * re;
 }
 
-// ---- FP32 storage on a cubic grid of N^3 velocities (N from fsm_pair_size): the forward transforms
+// ---- FP32 storage on a cubic grid of N^3 velocities (N from fsm_cube_size): the forward transforms
 // in LDS, no product array.  One workgroup per (cell, pair): pair p < nd transforms f_spec phi_p and
 // f_spec psi_p, pair nd transforms f_spec phipsi and f_spec; the product Re(A B) of the two
 // transforms is written per pair (part[cell][pair][n3]) and k_fsm_sum adds the pairs in a fixed
@@ -149,7 +151,7 @@ __global__ __launch_bounds__(fsm_pair_threads<N>()) void k_fsm_pair(const float2
                 const float m = (mode ? mc[N2 * k] : 1.0f) * inv;
                 x[k] = make_float2(x[k].x * m, x[k].y * m);
             }
-            fft_dit<N, N, 1>(x, X, tw);
+            fft_dit<N, N, 1, float2>(x, X, tw);
             float2* r = lds + ta + P * tb;
 #pragma unroll
             for (int k = 0; k < N; k++) r[P * N * k] = X[k];
@@ -159,7 +161,7 @@ __global__ __launch_bounds__(fsm_pair_threads<N>()) void k_fsm_pair(const float2
             float2* r = lds + P * (ta + N * tb);
 #pragma unroll
             for (int k = 0; k < N; k++) x[k] = r[k];
-            fft_dit<N, N, 1>(x, X, tw);
+            fft_dit<N, N, 1, float2>(x, X, tw);
 #pragma unroll
             for (int k = 0; k < N; k++) r[k] = X[k];
         }
@@ -168,7 +170,7 @@ __global__ __launch_bounds__(fsm_pair_threads<N>()) void k_fsm_pair(const float2
             const float2* r = lds + ta + P * N * tb;
 #pragma unroll
             for (int k = 0; k < N; k++) x[k] = r[P * k];
-            fft_dit<N, N, 1>(x, X, tw);
+            fft_dit<N, N, 1, float2>(x, X, tw);
             if (h == 0) {
 #pragma unroll
                 for (int k = 0; k < N; k++) A[k] = X[k];
@@ -178,6 +180,74 @@ __global__ __launch_bounds__(fsm_pair_threads<N>()) void k_fsm_pair(const float2
                 for (int k = 0; k < N; k++) o[N * k] = A[k].x * X[k].x - A[k].y * X[k].y;
             }
         }
+    }
+}
+
+// ---- FP64 storage on a cubic grid: the transforms of the products f_spec phi_d, f_spec psi_d,
+// f_spec phipsi, f_spec in the layout of k_fsm_modes (work[cell][2 M(M-1) + 2][n3]), each written
+// once by k_fsm_slab64, then k_fsm_q.  One workgroup per slab of S w-planes of one product array:
+//   1. every w-pencil (u, v) of the array: f_spec times the mode straight from the backward
+//      transform's output (the fftshift is an index map), transformed along w in registers; the S
+//      frequencies of the slab go to LDS (rows padded to an odd length).  The N / S slabs of an array
+//      each redo these transforms: arithmetic from L2-resident inputs instead of a pass over HBM;
+//   2. one thread per u-pencil of the slab, in LDS;
+//   3. one thread per v-pencil, whose transform goes straight to the array.
+template <int N> constexpr int fsm_slab() { return N % 8 == 0 ? 8 : 6; }
+template <int N> constexpr int fsm_slab_threads() { return (fsm_slab<N>() * N + 63) / 64 * 64; }
+template <int N> constexpr int fsm_slab_lds() { return fsm_slab<N>() * N * (N | 1) * 16; }
+
+template <int N>
+__global__ __launch_bounds__(fsm_slab_threads<N>()) void k_fsm_slab64(const double2* __restrict__ spec, const double* __restrict__ phi,
+                                                                    const double* __restrict__ psi, const double* __restrict__ phipsi,
+                                                                    const double2* __restrict__ tw, double2* __restrict__ work, int nd,
+                                                                    int ncell) {
+    constexpr int N2 = N * N, N3 = N2 * N, P = N | 1, H = N / 2, S = fsm_slab<N>(), NS = N / S;
+    constexpr int NTH = fsm_slab_threads<N>();
+    extern __shared__ double2 slab[];
+    const int na = 2 * nd + 2;
+    const int item = fsm_xcd_item();
+    if (item >= ncell * na * NS) return;
+    const int sl = item % NS, a = (item / NS) % na, ci = item / (NS * na);
+    const int p = a >> 1, h = a & 1;
+    const double* __restrict__ mode = p < nd ? (h ? psi : phi) + (size_t)p * N3 : (h ? nullptr : phipsi);
+    const double inv = 1.0 / double(N3);
+    const int t = threadIdx.x;
+    double2 x[N], X[N];
+    // 1. along w, every pencil of the array; the slab's planes kept
+    for (int pen = t; pen < N2; pen += NTH) {
+        const int ta = pen % N, tb = pen / N;
+        const double2* __restrict__ col = spec + (size_t)ci * N3 + (ta + N - H) % N + N * ((tb + N - H) % N);
+#pragma unroll
+        for (int k = 0; k < N; k++) {
+            const double2 z = col[N2 * ((k + N - H) % N)];
+            const double m = mode ? mode[pen + N2 * k] : 1.0;
+            x[k] = make_double2((z.x * inv) * m, (z.y * inv) * m);
+        }
+        fft_dit<N, N, 1, double2>(x, X, tw);
+#pragma unroll
+        for (int k = 0; k < N; k++)  // compile-time register indices, a uniform branch per plane
+            if (k / S == sl) slab[ta + P * (tb + N * (k % S))] = X[k];
+    }
+    __syncthreads();
+    const bool act = t < S * N;
+    const int ta = t % N, tb = t / N;
+    if (act) {  // 2. along u: pencil (v, w) = (ta, tb)
+        double2* r = slab + P * (ta + N * tb);
+#pragma unroll
+        for (int k = 0; k < N; k++) x[k] = r[k];
+        fft_dit<N, N, 1, double2>(x, X, tw);
+#pragma unroll
+        for (int k = 0; k < N; k++) r[k] = X[k];
+    }
+    __syncthreads();
+    if (act) {  // 3. along v: pencil (u, w) = (ta, tb), to the array
+        const double2* r = slab + ta + P * N * tb;
+#pragma unroll
+        for (int k = 0; k < N; k++) x[k] = r[P * k];
+        fft_dit<N, N, 1, double2>(x, X, tw);
+        double2* __restrict__ o = work + ((size_t)ci * na + a) * N3 + ta + N2 * (sl * S + tb);
+#pragma unroll
+        for (int k = 0; k < N; k++) o[N * k] = X[k];
     }
 }
 
@@ -202,8 +272,19 @@ template <int N> hipError_t launch_pair(const FsmPlan& P, hipStream_t st) {
     return hipGetLastError();
 }
 
-template <int N> hipError_t pair_attr() {
-    return hipFuncSetAttribute((const void*)k_fsm_pair<N>, hipFuncAttributeMaxDynamicSharedMemorySize, fsm_pair_lds<N>());
+template <int N> hipError_t launch_cube64(const FsmPlan& P, hipStream_t st) {
+    const int nd = P.M * (P.M - 1), na = 2 * nd + 2;
+    const unsigned blocks = unsigned(P.ncell * na * (N / fsm_slab<N>()));
+    hipLaunchKernelGGL(k_fsm_slab64<N>, dim3(blocks), dim3(fsm_slab_threads<N>()), fsm_slab_lds<N>(), st, (const double2*)P.spec,
+                       (const double*)P.phi, (const double*)P.psi, (const double*)P.phipsi, (const double2*)P.tw,
+                       (double2*)P.work, nd, P.ncell);
+    return hipGetLastError();
+}
+
+template <int N> hipError_t cube_attr() {
+    hipError_t e = hipFuncSetAttribute((const void*)k_fsm_pair<N>, hipFuncAttributeMaxDynamicSharedMemorySize, fsm_pair_lds<N>());
+    if (e != hipSuccess) return e;
+    return hipFuncSetAttribute((const void*)k_fsm_slab64<N>, hipFuncAttributeMaxDynamicSharedMemorySize, fsm_slab_lds<N>());
 }
 
 unsigned blocks_of(size_t n) { return unsigned((n + 255) / 256); }
@@ -227,8 +308,8 @@ template <class R> hipError_t fsm_run(const FsmPlan& P, bool fp64, const void* f
     if (exec(P, P.plan_b, spec, HIPFFT_BACKWARD) != HIPFFT_SUCCESS) return hipErrorUnknown;
     const double scale = 4.0 * kPi * kPi / P.kn / (double(P.M) * P.M);
     if constexpr (sizeof(R) == 4) {
-        if (P.pair_n) {
-            switch (P.pair_n) {
+        if (P.cube_n) {
+            switch (P.cube_n) {
             case 8: e = launch_pair<8>(P, st); break;
             case 12: e = launch_pair<12>(P, st); break;
             case 16: e = launch_pair<16>(P, st); break;
@@ -240,19 +321,30 @@ template <class R> hipError_t fsm_run(const FsmPlan& P, bool fp64, const void* f
             return hipGetLastError();
         }
     }
-    if (hipfftSetStream(P.plan_f, st) != HIPFFT_SUCCESS) return hipErrorUnknown;
-    hipLaunchKernelGGL(k_fsm_modes<R>, g, b, 0, st, spec, work, (const R*)P.phi, (const R*)P.psi, (const R*)P.phipsi, P.nu,
-                       P.nv, P.nw, nd, P.ncell);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    if (exec(P, P.plan_f, work, HIPFFT_FORWARD) != HIPFFT_SUCCESS) return hipErrorUnknown;
+    if (P.cube_n) {
+        switch (P.cube_n) {
+        case 8: e = launch_cube64<8>(P, st); break;
+        case 12: e = launch_cube64<12>(P, st); break;
+        case 16: e = launch_cube64<16>(P, st); break;
+        case 18: e = launch_cube64<18>(P, st); break;
+        default: e = launch_cube64<24>(P, st); break;
+        }
+        if (e != hipSuccess) return e;
+    } else {
+        if (hipfftSetStream(P.plan_f, st) != HIPFFT_SUCCESS) return hipErrorUnknown;
+        hipLaunchKernelGGL(k_fsm_modes<R>, g, b, 0, st, spec, work, (const R*)P.phi, (const R*)P.psi, (const R*)P.phipsi, P.nu,
+                           P.nv, P.nw, nd, P.ncell);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        if (exec(P, P.plan_f, work, HIPFFT_FORWARD) != HIPFFT_SUCCESS) return hipErrorUnknown;
+    }
     hipLaunchKernelGGL(k_fsm_q<R>, g, b, 0, st, work, P.Q, n3, nd, P.ncell, scale);
     return hipGetLastError();
 }
 
 }  // namespace
 
-int fsm_pair_size(bool f32, int nu, int nv, int nw) {
-    if (!f32 || nu != nv || nu != nw) return 0;
+int fsm_cube_size(int nu, int nv, int nw) {
+    if (nu != nv || nu != nw) return 0;
     for (int n : {8, 12, 16, 18, 24})
         if (n == nu) return n;
     return 0;
@@ -267,19 +359,19 @@ hipError_t fsm_plan_create(FsmPlan& P) {
     const int n3 = P.nu * P.nv * P.nw, nd = P.M * (P.M - 1);
     int n[3] = {P.nw, P.nv, P.nu};
     const hipfftType t = P.f32 ? HIPFFT_C2C : HIPFFT_Z2Z;
-    if (P.pair_n) {
+    if (P.cube_n) {
         hipError_t e;
-        switch (P.pair_n) {
-        case 8: e = pair_attr<8>(); break;
-        case 12: e = pair_attr<12>(); break;
-        case 16: e = pair_attr<16>(); break;
-        case 18: e = pair_attr<18>(); break;
-        default: e = pair_attr<24>(); break;
+        switch (P.cube_n) {
+        case 8: e = cube_attr<8>(); break;
+        case 12: e = cube_attr<12>(); break;
+        case 16: e = cube_attr<16>(); break;
+        case 18: e = cube_attr<18>(); break;
+        default: e = cube_attr<24>(); break;
         }
         if (e != hipSuccess) return e;
     }
     if (hipfftPlanMany(&P.plan_b, 3, n, nullptr, 1, n3, nullptr, 1, n3, t, P.ncell) != HIPFFT_SUCCESS) return hipErrorUnknown;
-    if (!P.pair_n &&
+    if (!P.cube_n &&
         hipfftPlanMany(&P.plan_f, 3, n, nullptr, 1, n3, nullptr, 1, n3, t, P.ncell * (2 * nd + 2)) != HIPFFT_SUCCESS) {
         hipfftDestroy(P.plan_b);
         return hipErrorUnknown;
@@ -291,7 +383,7 @@ hipError_t fsm_plan_create(FsmPlan& P) {
 void fsm_plan_destroy(FsmPlan& P) {
     if (!P.planned) return;
     hipfftDestroy(P.plan_b);
-    if (!P.pair_n) hipfftDestroy(P.plan_f);
+    if (!P.cube_n) hipfftDestroy(P.plan_f);
     P.planned = false;
 }
 

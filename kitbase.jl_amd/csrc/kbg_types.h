@@ -186,9 +186,10 @@ struct FsmPlan {
     void* psi = nullptr;        // real [M(M-1)][n3]
     void* phipsi = nullptr;     // real [n3]
     double* Q = nullptr;        // [ncell][n3]
-    // FP32 cubes of side pair_n (fsm_pair_size, else 0): forward transforms in LDS, per-pair
-    // products part [ncell][M(M-1) + 1][n3], twiddles tw[pair_n] = exp(-2 pi i j / pair_n); no work
-    int pair_n = 0;
+    // cubic grids of side cube_n (fsm_cube_size, else 0) take the in-house forward transforms, with
+    // twiddles tw[cube_n] = exp(-2 pi i j / cube_n) in the plan's precision: FP32 in LDS per direction
+    // pair (products part [ncell][M(M-1) + 1][n3], no work), FP64 through work in three passes
+    int cube_n = 0;
     float* part = nullptr;
     void* tw = nullptr;
     struct hipfftHandle_t* plan_b = nullptr;
@@ -198,8 +199,8 @@ struct FsmPlan {
 // Q of the plan's cells from the f buffer f (cell stride fcs, storage index of cell c0 = k0)
 hipError_t fsm_collision(const FsmPlan& P, bool fp64, const void* f, long fcs, int k0, hipStream_t st);
 hipError_t fsm_plan_create(FsmPlan& P);
-// side of the cubic velocity grids whose FP32 forward transforms run in LDS (8, 12, 16, 18, 24), else 0
-int fsm_pair_size(bool f32, int nu, int nv, int nw);
+// side of the cubic velocity grids whose forward transforms are the library's own (8, 12, 16, 18, 24), else 0
+int fsm_cube_size(int nu, int nv, int nw);
 void fsm_plan_destroy(FsmPlan& P);
 hipError_t launch_resid1d(const Solve1D& S, const double* d_rcell, int nrows, int boundary_split, double* d_res,
                           hipStream_t st);

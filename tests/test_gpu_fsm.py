@@ -86,9 +86,10 @@ def homogeneous_bimodal(ks, ctr):
 
 
 @pytest.mark.parametrize("precision,tol,shape", [
-    ("fp64", 1e-10, (16, 16, 16)), ("fp64", 1e-10, (16, 12, 12)),
-    # FP32 cubes of side 8, 12, 16, 18, 24 transform in LDS (one kernel instance per side); other
-    # shapes through hipFFT in FP32
+    # cubes of side 8, 12, 16, 18, 24 take the library's own forward transforms (one kernel instance
+    # per side and precision: FP64 through HBM in three passes, FP32 in LDS); other shapes hipFFT
+    ("fp64", 1e-10, (8, 8, 8)), ("fp64", 1e-10, (12, 12, 12)), ("fp64", 1e-10, (16, 16, 16)),
+    ("fp64", 1e-10, (18, 18, 18)), ("fp64", 1e-10, (24, 24, 24)), ("fp64", 1e-10, (16, 12, 12)),
     ("fp32", 1e-4, (8, 8, 8)), ("fp32", 1e-4, (12, 12, 12)), ("fp32", 1e-4, (16, 16, 16)),
     ("fp32", 1e-4, (18, 18, 18)), ("fp32", 1e-4, (24, 24, 24)), ("fp32", 1e-4, (16, 12, 12))])
 def test_fsm_homogeneous_step_is_dt_q(precision, tol, shape):
