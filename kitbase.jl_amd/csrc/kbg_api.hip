@@ -12,6 +12,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -86,6 +87,11 @@ struct kbg_ctx {
         }
         gsig.clear();
     }
+    // host <-> device transfers of f: two pinned host buffers (hipHostMalloc) that worker threads
+    // fill or drain while the other one's copy runs, allocated at the first large transfer
+    static constexpr size_t kPinBytes = size_t(32) << 20;
+    void* pin[2] = {};
+    hipEvent_t pin_ev[2] = {};
 };
 
 struct kbg_solve {
@@ -136,31 +142,99 @@ int dev_alloc(kbg_solve* s, void** p, size_t bytes) {
     return KBG_OK;
 }
 
+// f(i0, i1) over [0, n) split across up to 8 host threads (one below 1 M elements)
+template <class F> void par_for(size_t n, F&& f) {
+    static const int hw = std::max(1, std::min(8, (int)std::thread::hardware_concurrency()));
+    const int T = n < (size_t(1) << 20) ? 1 : hw;
+    if (T == 1) {
+        f(size_t(0), n);
+        return;
+    }
+    const size_t per = (n + T - 1) / T;
+    std::vector<std::thread> th;
+    for (int t = 1; t < T; t++) {
+        const size_t a = t * per, b = std::min(n, a + per);
+        if (a < b) th.emplace_back([&f, a, b] { f(a, b); });
+    }
+    f(size_t(0), std::min(n, per));
+    for (auto& x : th) x.join();
+}
+
+int pin_ready(kbg_ctx* c) {
+    for (int b = 0; b < 2; b++) {
+        if (!c->pin[b]) HIPCHK(hipHostMalloc(&c->pin[b], kbg_ctx::kPinBytes, hipHostMallocDefault));
+        if (!c->pin_ev[b]) HIPCHK(hipEventCreateWithFlags(&c->pin_ev[b], hipEventDisableTiming));
+    }
+    return KBG_OK;
+}
+
+// dst[i] = conv(i) for i < n on the device (T the device type): chunks converted by worker threads
+// into one pinned buffer while the other one's copy runs
+template <class T, class Conv> int stage_up(kbg_solve* s, T* dst, size_t n, Conv conv) {
+    kbg_ctx* c = s->ctx;
+    int rc;
+    if ((rc = pin_ready(c))) return rc;
+    const size_t per = kbg_ctx::kPinBytes / sizeof(T);
+    for (size_t k = 0, i0 = 0; i0 < n; k++, i0 += per) {
+        const int b = int(k & 1);
+        const size_t m = std::min(per, n - i0);
+        if (k >= 2) HIPCHK(hipEventSynchronize(c->pin_ev[b]));  // this buffer's previous copy is done
+        T* hb = static_cast<T*>(c->pin[b]);
+        par_for(m, [&](size_t a, size_t e) {
+            for (size_t i = a; i < e; i++) hb[i] = conv(i0 + i);
+        });
+        HIPCHK(hipMemcpyAsync(dst + i0, hb, m * sizeof(T), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipEventRecord(c->pin_ev[b], c->stream));
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return KBG_OK;
+}
+
+// store(i, src[i]) for i < n from the device: the next chunk's copy runs while worker threads drain
+// the current one
+template <class T, class Store> int stage_down(kbg_solve* s, const T* src, size_t n, Store store) {
+    kbg_ctx* c = s->ctx;
+    int rc;
+    if ((rc = pin_ready(c))) return rc;
+    const size_t per = kbg_ctx::kPinBytes / sizeof(T), nch = (n + per - 1) / per;
+    auto issue = [&](size_t k) -> int {
+        const int b = int(k & 1);
+        const size_t i0 = k * per, m = std::min(per, n - i0);
+        HIPCHK(hipMemcpyAsync(c->pin[b], src + i0, m * sizeof(T), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipEventRecord(c->pin_ev[b], c->stream));
+        return KBG_OK;
+    };
+    if (nch && (rc = issue(0))) return rc;
+    for (size_t k = 0; k < nch; k++) {
+        if (k + 1 < nch && (rc = issue(k + 1))) return rc;  // into the buffer chunk k - 1 was drained from
+        const int b = int(k & 1);
+        HIPCHK(hipEventSynchronize(c->pin_ev[b]));
+        const size_t i0 = k * per, m = std::min(per, n - i0);
+        const T* hb = static_cast<const T*>(c->pin[b]);
+        par_for(m, [&](size_t a, size_t e) {
+            for (size_t i = a; i < e; i++) store(i0 + i, hb[i]);
+        });
+    }
+    return KBG_OK;
+}
+
 // copy n doubles to device as the solve's storage type
 int put_vec(kbg_solve* s, void* dst, const double* src, size_t n) {
     if (s->fp64) {
         HIPCHK(hipMemcpyAsync(dst, src, n * 8, hipMemcpyHostToDevice, s->ctx->stream));
         HIPCHK(hipStreamSynchronize(s->ctx->stream));
-    } else {
-        std::vector<float> tmp(n);
-        for (size_t i = 0; i < n; i++) tmp[i] = float(src[i]);
-        HIPCHK(hipMemcpyAsync(dst, tmp.data(), n * 4, hipMemcpyHostToDevice, s->ctx->stream));
-        HIPCHK(hipStreamSynchronize(s->ctx->stream));
+        return KBG_OK;
     }
-    return KBG_OK;
+    return stage_up(s, static_cast<float*>(dst), n, [src](size_t i) { return float(src[i]); });
 }
 
 int get_vec(kbg_solve* s, double* dst, const void* src, size_t n) {
     if (s->fp64) {
         HIPCHK(hipMemcpyAsync(dst, src, n * 8, hipMemcpyDeviceToHost, s->ctx->stream));
         HIPCHK(hipStreamSynchronize(s->ctx->stream));
-    } else {
-        std::vector<float> tmp(n);
-        HIPCHK(hipMemcpyAsync(tmp.data(), src, n * 4, hipMemcpyDeviceToHost, s->ctx->stream));
-        HIPCHK(hipStreamSynchronize(s->ctx->stream));
-        for (size_t i = 0; i < n; i++) dst[i] = double(tmp[i]);
+        return KBG_OK;
     }
-    return KBG_OK;
+    return stage_down(s, static_cast<const float*>(src), n, [dst](size_t i, float v) { dst[i] = double(v); });
 }
 
 int put_f64(kbg_solve* s, double* dst, const double* src, size_t n) {
@@ -711,6 +785,10 @@ int kbg_ctx_destroy(kbg_ctx* c) {
     (void)hipFree(c->d_res);
     (void)hipFree(c->d_descs2);
     (void)hipFree(c->d_pref2);
+    for (int b = 0; b < 2; b++) {
+        if (c->pin[b]) (void)hipHostFree(c->pin[b]);
+        if (c->pin_ev[b]) (void)hipEventDestroy(c->pin_ev[b]);
+    }
     for (auto& sd : c->side)
         if (sd) (void)hipStreamDestroy(sd);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1300,11 +1378,11 @@ int kbg_upload(kbg_solve* s, const double* w, const double* prim, const double* 
     // departures from a reference Maxwellian of this state; the face fluxes count as full values
     set_mref2d(s, w);
     const size_t RS = 2 * (size_t)s->d.nv, FX = size_t(s->d.nx + 1) * s->d.ny, FY = size_t(s->d.nx) * (s->d.ny + 1);
-    std::vector<float> tmp(N * RS);
-    for (size_t c = 0; c < N; c++)
-        for (size_t k = 0; k < RS; k++) tmp[c * RS + k] = float(f[c * RS + k] - double(s->mref[k]));
+    const float* mr = s->mref.data();
+    const double* fs = f;
+    if ((rc = stage_up(s, static_cast<float*>(S.f), N * RS, [fs, mr, RS](size_t i) { return float(fs[i] - double(mr[i % RS])); })))
+        return rc;
     HIPCHK(hipMemcpyAsync((void*)S.mref, s->mref.data(), RS * 4, hipMemcpyHostToDevice, s->ctx->stream));
-    HIPCHK(hipMemcpyAsync(S.f, tmp.data(), N * RS * 4, hipMemcpyHostToDevice, s->ctx->stream));
     HIPCHK(hipMemsetAsync(S.xcref, 0, FX * 8, s->ctx->stream));
     HIPCHK(hipMemsetAsync(S.ycref, 0, FY * 8, s->ctx->stream));
     HIPCHK(hipStreamSynchronize(s->ctx->stream));
@@ -1346,12 +1424,15 @@ int kbg_download(kbg_solve* s, double* w, double* prim, double* f) {
         wide.resize(N * 2 * s->d.nv);
         fo = wide.data();
     }
-    if ((rc = get_vec(s, fo, S.f, N * 2 * s->d.nv))) return rc;
     if (S.delta && !s->rep_full) {
         // departure + Mr, exactly (two FP32 values summed in FP64)
         const size_t RS = 2 * (size_t)s->d.nv;
-        for (size_t c = 0; c < N; c++)
-            for (size_t k = 0; k < RS; k++) fo[c * RS + k] += double(s->mref[k]);
+        const float* mr = s->mref.data();
+        if ((rc = stage_down(s, static_cast<const float*>(S.f), N * RS,
+                             [fo, mr, RS](size_t i, float v) { fo[i] = double(v) + double(mr[i % RS]); })))
+            return rc;
+    } else if ((rc = get_vec(s, fo, S.f, N * 2 * s->d.nv))) {
+        return rc;
     }
     if (s->d.nf == 1) narrow2d(s, wide, f, N);
     return KBG_OK;
