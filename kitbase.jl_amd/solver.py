@@ -23,6 +23,7 @@ Velocity arrays are flattened u-fastest (Julia column-major).
 from __future__ import annotations
 
 import ctypes as C
+import os
 from dataclasses import dataclass, field
 from pathlib import Path
 from typing import Any, Callable
@@ -315,6 +316,22 @@ class Interfaces:
     yff: np.ndarray | None = None
 
 
+def _gather_rows(table: np.ndarray, idx: np.ndarray) -> np.ndarray:
+    """table[idx] as a new array; large ones are filled by up to 8 threads (numpy releases the
+    interpreter lock while it copies), which also spreads the first touch of the pages."""
+    out = np.empty((idx.size,) + table.shape[1:], dtype=table.dtype)
+    if out.nbytes < (64 << 20):
+        np.take(table, idx, axis=0, out=out)
+        return out
+    from concurrent.futures import ThreadPoolExecutor
+
+    nt = max(1, min(8, os.cpu_count() or 1))
+    cuts = np.linspace(0, idx.size, nt + 1).astype(int)
+    with ThreadPoolExecutor(nt) as ex:
+        list(ex.map(lambda k: np.take(table, idx[cuts[k]:cuts[k + 1]], axis=0, out=out[cuts[k]:cuts[k + 1]]), range(nt)))
+    return out
+
+
 def init_fvm(ks: SolverSet) -> tuple[ControlVolumes, Interfaces]:
     """Cell and face arrays with KitBase's initial values (solver_init.jl:62-327)."""
     g = ks.gas.gamma
@@ -347,20 +364,20 @@ def init_fvm(ks: SolverSet) -> tuple[ControlVolumes, Interfaces]:
     ps2: PSpace2D = ks.ps
     nx, ny, nf = ps2.nx, ps2.ny, ks.nf
     w = np.empty((ny, nx, 4))
-    f = np.empty((ny, nx, nf, nv))
-    seen2: dict[bytes, tuple[int, int]] = {}  # piecewise-constant initial states: evaluate each once
+    # piecewise-constant initial states: each distinct state evaluated once, then gathered into f
+    seen2: dict[bytes, int] = {}
+    rows: list[np.ndarray] = []
+    idx = np.empty(ny * nx, dtype=np.intp)
     for j in range(ny):
         for i in range(nx):
             w[j, i] = ks.ib.fw(ps2.x[j, i], ps2.y[j, i])
             key = w[j, i].tobytes()
-            if key in seen2:
-                f[j, i] = f[seen2[key]]
-                continue
-            seen2[key] = (j, i)
-            if nf == 1:
-                f[j, i, 0] = ks.ib.ff(ps2.x[j, i], ps2.y[j, i])
-            else:
-                f[j, i, 0], f[j, i, 1] = ks.ib.ff(ps2.x[j, i], ps2.y[j, i])
+            if key not in seen2:
+                seen2[key] = len(rows)
+                ff = ks.ib.ff(ps2.x[j, i], ps2.y[j, i])
+                rows.append(np.asarray(ff, dtype=np.float64).reshape(nf, nv))
+            idx[j * nx + i] = seen2[key]
+    f = _gather_rows(np.stack(rows), idx).reshape(ny, nx, nf, nv)
     prim = conserve_prim(w, g)
     ctr = ControlVolumes(w, prim, f, np.zeros((ny, nx, 2, 4)), np.zeros((ny, nx, 2, nf, nv)))
     # Interface(W, H, B, ND) keeps fw = deepcopy(W) (struct_face.jl:96); Interface(W, F, ND) starts at 0
