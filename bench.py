@@ -244,16 +244,22 @@ def serving_leg(kb, ctx, n, rates, chunk, overlap):
     d3.step(kb.timestep(ks3, c3), 1)
     d3.close()
     out["warmup_s"] = time.perf_counter() - t0
+    # uploads and downloads through a second context while the chunks run on ctx
+    xctx = kb.Context(0) if overlap else None
+    xfer = kb.serving.DeviceBackend(kb, ctx=xctx) if overlap else None
+    out["transfer_context"] = bool(overlap)
     for rate in rates:
         reqs = [kb.serving.Request(rid=k, cfg=cfg, nsteps=ns, arrival=t)
                 for k, (kind, cfg, ns, t) in enumerate(kb.configs.serving_requests(n, rate))]
         srv = kb.serving.Server(kb.serving.DeviceBackend(kb, ctx=ctx), budget_bytes=32 << 30, chunk_steps=chunk,
-                                overlap=overlap)
+                                overlap=overlap, transfer=xfer)
         st = srv.run(reqs).summary()
         out["rates"][str(rate)] = st
         log(f"[bench] serving {rate:g}/s: {st['requests']} requests, p50 {st['p50_ms']:.1f} ms, p99 {st['p99_ms']:.1f} ms, "
             f"{st['updates_per_s']:.3e} updates/s, max live {st['max_live']}")
     ctx.set_sweep_vw(0)
+    if xctx is not None:
+        xctx.close()
     return out
 
 

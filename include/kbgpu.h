@@ -216,7 +216,11 @@ int kbg_step(kbg_solve *s, double dt, int nsteps, double *residual);
 
 /* One solve!-loop iteration for every solve of a batch, nsteps times: the 1D solves share one
  * fused launch per step; the others are stepped in turn on the same stream.  dts[i] is solve i's
- * time step; residuals (n x 5, may be NULL) receive the last step's residuals. */
+ * time step; residuals (n x 5, may be NULL) receive the last step's residuals.  The solves may have
+ * been created on any context of ctx's device: the batch runs on ctx's streams, while a solve's own
+ * calls (upload, download, phase calls, destroy) run on its creating context's stream, so another
+ * host thread may upload or download solves of a second context while the batch runs, as long as
+ * none of them is in the batch (a serving loop overlaps its transfers with the stepping this way). */
 int kbg_batch_step(kbg_ctx *ctx, kbg_solve *const *solves, int n, const double *dts, int nsteps,
                    double *residuals);
 

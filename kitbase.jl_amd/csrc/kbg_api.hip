@@ -439,13 +439,16 @@ template <class T> void build_vtab(const kbg_desc& d, double dt, std::vector<cha
     }
 }
 
-int ensure_vtab(kbg_solve* s, double dt) {
+// st: the stream of the calling batch (a solve may belong to another context of the device), or
+// nullptr for the solve's own context stream
+int ensure_vtab(kbg_solve* s, double dt, hipStream_t st = nullptr) {
     if (!s->fused || s->vtab_dt == dt) return KBG_OK;
     std::vector<char> tab;
     if (s->fp64) build_vtab<double>(s->d, dt, tab);
     else build_vtab<float>(s->d, dt, tab);
-    HIPCHK(hipMemcpyAsync(s->vtab, tab.data(), tab.size(), hipMemcpyHostToDevice, s->ctx->stream));
-    HIPCHK(hipStreamSynchronize(s->ctx->stream));
+    const hipStream_t q = st ? st : s->ctx->stream;
+    HIPCHK(hipMemcpyAsync(s->vtab, tab.data(), tab.size(), hipMemcpyHostToDevice, q));
+    HIPCHK(hipStreamSynchronize(q));
     s->vtab_dt = dt;
     return KBG_OK;
 }
@@ -591,21 +594,25 @@ void fill2d(kbg_solve* s) {
     S.dt = 0.0;
 }
 
-int push1d(kbg_solve* s) {
-    HIPCHK(hipMemcpyAsync(s->d1, &s->h1, sizeof(Solve1D), hipMemcpyHostToDevice, s->ctx->stream));
-    HIPCHK(hipStreamSynchronize(s->ctx->stream));
+// the solve's device descriptor from its host copy, on st (nullptr: the solve's context stream)
+int push1d(kbg_solve* s, hipStream_t st = nullptr) {
+    const hipStream_t q = st ? st : s->ctx->stream;
+    HIPCHK(hipMemcpyAsync(s->d1, &s->h1, sizeof(Solve1D), hipMemcpyHostToDevice, q));
+    HIPCHK(hipStreamSynchronize(q));
     return KBG_OK;
 }
 
-int push2d(kbg_solve* s) {
-    HIPCHK(hipMemcpyAsync(s->d2, &s->h2, sizeof(Solve2D), hipMemcpyHostToDevice, s->ctx->stream));
-    HIPCHK(hipStreamSynchronize(s->ctx->stream));
+int push2d(kbg_solve* s, hipStream_t st = nullptr) {
+    const hipStream_t q = st ? st : s->ctx->stream;
+    HIPCHK(hipMemcpyAsync(s->d2, &s->h2, sizeof(Solve2D), hipMemcpyHostToDevice, q));
+    HIPCHK(hipStreamSynchronize(q));
     return KBG_OK;
 }
 
-int pushus(kbg_solve* s) {
-    HIPCHK(hipMemcpyAsync(s->dus, &s->hus, sizeof(kbg::SolveUS), hipMemcpyHostToDevice, s->ctx->stream));
-    HIPCHK(hipStreamSynchronize(s->ctx->stream));
+int pushus(kbg_solve* s, hipStream_t st = nullptr) {
+    const hipStream_t q = st ? st : s->ctx->stream;
+    HIPCHK(hipMemcpyAsync(s->dus, &s->hus, sizeof(kbg::SolveUS), hipMemcpyHostToDevice, q));
+    HIPCHK(hipStreamSynchronize(q));
     return KBG_OK;
 }
 
@@ -1671,7 +1678,7 @@ int kbg_batch_timestep(kbg_ctx* ctx, kbg_solve* const* solves, int n, const doub
     // one reduction per solve on the context stream, each into its solve's result slot; one sync
     for (int i = 0; i < n; i++) {
         kbg_solve* s = solves[i];
-        if (!s || s->ctx != ctx) return fail(KBG_ERR_ARG, "solve " + std::to_string(i) + " is NULL or from another context");
+        if (!s || s->ctx->device != ctx->device) return fail(KBG_ERR_ARG, "solve " + std::to_string(i) + " is NULL or on another device");
         if (!(cfl[i] > 0.0)) return fail(KBG_ERR_ARG, "cfl must be positive");
         if (s->us) {
             if (!s->dx) return fail(KBG_ERR_ARG, "solve " + std::to_string(i) + ": unstructured, created without cell_dmin");
@@ -1732,10 +1739,10 @@ int batch_step_impl(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double*
     std::vector<int> fused, fused2, others;
     for (int i = 0; i < n; i++) {
         kbg_solve* s = solves[i];
-        if (!s || s->ctx != ctx) return fail(KBG_ERR_ARG, "solve is NULL or belongs to another context");
+        if (!s || s->ctx->device != ctx->device) return fail(KBG_ERR_ARG, "solve is NULL or on another device");
         if (s->d.dim == 1 && s->fused) {
             fused.push_back(i);
-            if ((rc = ensure_vtab(s, dts[i]))) return rc;
+            if ((rc = ensure_vtab(s, dts[i], st))) return rc;
         } else if (s->d.dim == 2 && s->thr2 > 0) {
             fused2.push_back(i);
         } else {
@@ -1824,7 +1831,7 @@ int batch_step_impl(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double*
         kbg_solve* s = solves[i];
         if (s->us) {
             s->hus.dt = dts[i];
-            if ((rc = pushus(s))) return rc;
+            if ((rc = pushus(s, st))) return rc;
             continue;
         }
         if (noalloc && s->has_fsm)
@@ -1836,10 +1843,10 @@ int batch_step_impl(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double*
         if ((rc = ensure_phase_buffers(s))) return rc;
         if (s->d.dim == 1) {
             s->h1.dt = dts[i];
-            if ((rc = push1d(s))) return rc;
+            if ((rc = push1d(s, st))) return rc;
         } else {
             s->h2.dt = dts[i];
-            if ((rc = push2d(s))) return rc;
+            if ((rc = push2d(s, st))) return rc;
         }
     }
     // descriptors, work items and packs; one launch per (stream, kernel instance)
@@ -1988,7 +1995,7 @@ int batch_step_impl(kbg_ctx* ctx, kbg_solve* const* solves, int n, const double*
     for (int i : fused2) {
         kbg_solve* s = solves[i];
         if (s->h2.delta && s->rep_full) {
-            if ((rc = push2d(s))) return rc;
+            if ((rc = push2d(s, st))) return rc;
             HIPCHK(kbg::launch_delta2d(s->h2, s->d2, 3, st));
             s->rep_full = false;
         }

@@ -85,13 +85,20 @@ class DeviceBackend:
     def step(self, handles, dts, nsteps: int):
         self.kb.batch_step(handles, dts, nsteps, ctx=self.ctx)
 
-    def finish(self, handle, prepared):
+    def download(self, handle, prepared):
         # fresh cell arrays for the result (the slope arrays of the prepared state stay zero and
         # are shared: a download does not touch them)
         p = prepared[1]
         ctr = dataclasses.replace(p, w=np.empty_like(p.w), prim=np.empty_like(p.prim), f=np.empty_like(p.f))
         handle.download(ctr)
+        return ctr
+
+    def release(self, handle):
         handle.close()
+
+    def finish(self, handle, prepared):
+        ctr = self.download(handle, prepared)
+        self.release(handle)
         return ctr
 
 
@@ -135,10 +142,17 @@ class ServeStats:
 class Server:
     """overlap: while a chunk runs on the device (backend.step on a worker thread; the library call
     releases the interpreter lock), the calling thread prepares the requests that arrive meanwhile
-    (host-side set-up only: no device calls), so host set-up does not stall the live requests."""
+    (host-side set-up only: no device calls), so host set-up does not stall the live requests.
+
+    transfer (with overlap): a second backend whose solves live on another context of the same
+    device (kbg_batch_step steps solves of any context of its device).  While a chunk runs, the
+    calling thread also uploads, through it, the waiting requests that fit (they join the next
+    chunk, the one they would have joined anyway) and downloads the requests the previous chunk
+    finished; their device memory is released between chunks.  The device then no longer idles
+    during transfers."""
 
     def __init__(self, backend, budget_bytes: int, chunk_steps: int = 25, clock=time.perf_counter, sleep=time.sleep,
-                 overlap: bool = False):
+                 overlap: bool = False, transfer=None):
         if chunk_steps < 1:
             raise ValueError("chunk_steps must be >= 1")
         self.backend = backend
@@ -147,8 +161,121 @@ class Server:
         self.clock = clock
         self.sleep = sleep
         self.overlap = overlap
+        self.transfer = transfer
 
     def run(self, requests: list[Request]) -> ServeStats:
+        if self.overlap and self.transfer is not None:
+            return self._run_transfer(requests)
+        return self._run(requests)
+
+    def _chunk(self, live):
+        """Steps of the next chunk and the requests it takes (pairs from each request's own start)."""
+        k = min(self.chunk, min(l.left for l in live))
+        if k == 1 and self.chunk > 1:
+            return 1, [l for l in live if l.left == 1]  # last odd steps, on their own
+        if k > 1:
+            k -= k & 1
+        return k, list(live)
+
+    def _run_transfer(self, requests: list[Request]) -> ServeStats:
+        stats = ServeStats()
+        pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
+        waiting: list[tuple[Request, Any, int]] = []
+        live: list[_Live] = []
+        ready: list[_Live] = []      # uploaded while a chunk ran: they join the next one
+        finished: list[_Live] = []   # done stepping, not yet downloaded
+        spent: list[_Live] = []      # downloaded, memory released between chunks
+        used = 0
+        xf = self.transfer
+        t0 = self.clock()
+        now = lambda: self.clock() - t0
+
+        def prepare_arrived():
+            while pending and pending[0].arrival <= now():
+                r = pending.pop(0)
+                h0 = self.clock()
+                prep = xf.prepare(r)
+                nb = xf.footprint(prep, r)
+                stats.host_s += self.clock() - h0
+                if nb > self.budget:
+                    raise ValueError(f"request {r.rid} needs {nb} bytes, more than the budget {self.budget}")
+                waiting.append((r, prep, nb))
+
+        def admit_one(into) -> bool:
+            nonlocal used
+            if not (waiting and used + waiting[0][2] <= self.budget):
+                return False
+            r, prep, nb = waiting.pop(0)
+            h0 = self.clock()
+            h, dt = xf.create(prep, r)
+            stats.host_s += self.clock() - h0
+            r.admitted = now()
+            into.append(_Live(r, h, dt, r.nsteps, nb, prep))
+            used += nb
+            return True
+
+        def download_one():
+            l = finished.pop(0)
+            h0 = self.clock()
+            l.req.result = xf.download(l.handle, l.prepared)
+            stats.host_s += self.clock() - h0
+            l.req.done = now()
+            stats.latencies_ms.append(1e3 * l.req.latency)
+            spent.append(l)
+
+        while pending or waiting or live or ready or finished or spent:
+            prepare_arrived()
+            live.extend(ready)
+            ready.clear()
+            for l in spent:
+                xf.release(l.handle)
+                used -= l.nbytes
+            spent.clear()
+            while admit_one(live):  # the device is between chunks: what fits joins the next one now
+                pass
+            stats.max_live = max(stats.max_live, len(live))
+            stats.peak_bytes = max(stats.peak_bytes, used)
+            if not live:
+                while finished:
+                    download_one()
+                if pending and not spent and not waiting:
+                    self.sleep(max(0.0, pending[0].arrival - now()))
+                continue
+            k, batch = self._chunk(live)
+            err = []
+
+            def run():
+                try:
+                    self.backend.step([l.handle for l in batch], [l.dt for l in batch], k)
+                except BaseException as e:  # re-raised on the calling thread
+                    err.append(e)
+
+            h0 = self.clock()
+            th = threading.Thread(target=run)
+            th.start()
+            while th.is_alive():
+                if pending and pending[0].arrival <= now():
+                    prepare_arrived()
+                elif finished:
+                    download_one()
+                elif not admit_one(ready):
+                    th.join(timeout=5e-4)
+            th.join()
+            if err:
+                raise err[0]
+            stats.step_s += self.clock() - h0
+            stats.chunks += 1
+            for l in batch:
+                l.left -= k
+                stats.updates += float(k) * l.req.points
+            for l in [l for l in live if l.left == 0]:
+                live.remove(l)
+                finished.append(l)
+            stats.peak_bytes = max(stats.peak_bytes, used)
+        stats.wall_s = now()
+        return stats
+
+    def _run(self, requests: list[Request]) -> ServeStats:
         stats = ServeStats()
         pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
         waiting: list[tuple[Request, Any, int]] = []

@@ -55,6 +55,32 @@ def test_each_result_is_the_same_alone_and_in_the_mix(gpu_ctx):
         assert np.array_equal(alone.w, mix.w) and np.array_equal(alone.prim, mix.prim), r.rid
 
 
+def test_transfers_on_a_second_context_give_the_same_results(gpu_ctx):
+    """Uploads and downloads through a second context while the chunks run on the first
+    (kbg_batch_step takes solves of any context of its device): each result still equals the
+    request run alone, bit for bit."""
+    reqs = _mix()
+    xctx = kb.Context(0)
+    step_be = kb.serving.DeviceBackend(kb, ctx=gpu_ctx)
+    xfer_be = kb.serving.DeviceBackend(kb, ctx=xctx)
+    srv = kb.serving.Server(step_be, budget_bytes=1 << 32, chunk_steps=7, overlap=True, transfer=xfer_be)
+    st = srv.run(reqs)
+    assert st.summary()["requests"] == len(reqs) and st.max_live >= 2
+    for r in reqs:
+        ks = kb.SolverSet(r.cfg)
+        ctr, face = kb.init_fvm(ks)
+        dt = kb.timestep(ks, ctr)
+        dev = kb.DeviceSolve(ks, ctr, face, precision=r.precision, ctx=gpu_ctx)
+        dev.step(dt, r.nsteps)
+        alone = copy.deepcopy(ctr)
+        dev.download(alone)
+        dev.close()
+        mix = r.result
+        assert np.array_equal(alone.f, mix.f), (r.rid, np.max(np.abs(alone.f - mix.f)))
+        assert np.array_equal(alone.w, mix.w) and np.array_equal(alone.prim, mix.prim), r.rid
+    xctx.close()
+
+
 def test_admission_waits_for_memory(gpu_ctx):
     """A budget that holds one request at a time still serves them all, one after another."""
     reqs = _mix()[:3]

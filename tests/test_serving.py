@@ -214,3 +214,62 @@ def test_overlap_prepares_arrivals_while_a_chunk_runs():
     assert done == {0: 30, 1: 20, 2: 10, 3: 10}
     assert be.prepared_during_step >= 1
     assert st.summary()["requests"] == 4 and not be.live
+
+
+def test_transfers_overlap_the_chunks_and_never_touch_a_running_batch():
+    """overlap with a transfer backend: uploads and downloads run while a chunk steps, never on a
+    request inside the running chunk; memory is released only between chunks; every request still
+    gets exactly its steps, in pairs from its own start, and its result."""
+    import threading
+    import time
+
+    class Device(FakeBackend):
+        def __init__(self):
+            super().__init__(clock=None)
+            self.running: set = set()
+            self.lock = threading.Lock()
+            self.overlapped = 0
+            self.violations = []
+
+        def step(self, handles, dts, nsteps):
+            assert set(handles) <= self.live
+            with self.lock:
+                self.running = set(handles)
+            self.calls.append((tuple(handles), nsteps))
+            time.sleep(0.01)
+            with self.lock:
+                self.running = set()
+
+        def check(self, what, h):
+            with self.lock:
+                if self.running:
+                    self.overlapped += 1
+                if h in self.running:
+                    self.violations.append((what, h))
+
+        def create(self, prep, r):
+            h, dt = super().create(prep, r)
+            self.check("create", h)
+            return h, dt
+
+        def download(self, h, prep):
+            self.check("download", h)
+            return ("state", h)
+
+        def release(self, h):
+            with self.lock:
+                if self.running:
+                    self.violations.append(("release during a chunk", h))
+            self.live.remove(h)
+
+    dev = Device()
+    reqs = [R(0, 30, 0.0), R(1, 20, 0.004), R(2, 10, 0.012), R(3, 13, 0.02), R(4, 8, 0.021, 60)]
+    srv = serving.Server(dev, 100, chunk_steps=6, overlap=True, transfer=dev)
+    st = srv.run(reqs)
+    assert not dev.violations and dev.overlapped > 0 and not dev.live
+    assert all(r.result == ("state", r.rid) for r in reqs) and st.summary()["requests"] == 5
+    for r in reqs:
+        seq = [n for h, n in dev.calls if r.rid in h]
+        assert sum(seq) == r.nsteps
+        assert all(n % 2 == 0 for n in seq[:-1]) and (seq[-1] % 2 == 0 or seq[-1] == 1), (r.rid, seq)
+    assert st.peak_bytes <= 100
