@@ -32,6 +32,7 @@ import argparse
 import re
 import copy
 import json
+import math
 import os
 import sys
 import time
@@ -223,7 +224,13 @@ def fsm_leg(kb, ctx, steps):
         t1 = time.perf_counter()
         d.step(dt, steps)
         wall_c = time.perf_counter() - t1
-        out[prec] = {"ms_per_step": wall_c * 1e3 / steps, "device_ms_per_step": ctx.last_steps_ms() / steps}
+        dev_ms = ctx.last_steps_ms() / steps
+        # the 3D transforms of a step (backward spectra + forward mode products) at 5 n log2 n flops each
+        n3 = ks.vs.nv
+        gflop = out["transforms_per_step"] * 5.0 * n3 * math.log2(n3) * 1e-9
+        out[prec] = {"ms_per_step": wall_c * 1e3 / steps, "device_ms_per_step": dev_ms,
+                     "transforms_per_s": out["transforms_per_step"] / (dev_ms * 1e-3),
+                     "fft_gflop_per_s": gflop / (dev_ms * 1e-3)}
         d.close()
     log(f"[bench] fsm: fp64 {out['fp64']['device_ms_per_step']:.3f} ms/step, fp32 {out['fp32']['device_ms_per_step']:.3f} ms/step")
     return out
