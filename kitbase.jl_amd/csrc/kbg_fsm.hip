@@ -117,7 +117,10 @@ KBG_D int fsm_xcd_item() {
 }
 
 template <int N> constexpr int fsm_pair_threads() { return (N * N + 63) / 64 * 64; }
-template <int N> constexpr int fsm_pair_lds() { return N * N * (N | 1) * 8; }
+// the first transform's kept v-pencil: its first KEEP values wait in LDS behind the array (registers
+// for the rest), which keeps the kernel within the register budget of 576-thread workgroups
+constexpr int kFsmKeep = 8;
+template <int N> constexpr int fsm_pair_lds() { return N * N * (N | 1) * 8 + kFsmKeep * N * N * 8; }
 
 template <int N>
 __global__ __launch_bounds__(fsm_pair_threads<N>()) void k_fsm_pair(const float2* __restrict__ spec, const float* __restrict__ phi,
@@ -137,6 +140,7 @@ __global__ __launch_bounds__(fsm_pair_threads<N>()) void k_fsm_pair(const float2
     // ((ta + N - N/2) % N, (tb + N - N/2) % N, (k + N - N/2) % N)
     const float2* __restrict__ col = spec + (size_t)ci * N3 + (ta + N - H) % N + N * ((tb + N - H) % N);
     const float inv = 1.0f / float(N3);
+    float2* keep = lds + N2 * P;  // [kFsmKeep][N2]: the first values of each thread's kept pencil
     float2 A[N];
     for (int h = 0; h < 2; h++) {
         const float* __restrict__ mode = p < nd ? (h ? psi : phi) + (size_t)p * N3 : (h ? nullptr : phipsi);
@@ -173,11 +177,17 @@ __global__ __launch_bounds__(fsm_pair_threads<N>()) void k_fsm_pair(const float2
             fft_dit<N, N, 1, float2>(x, X, tw);
             if (h == 0) {
 #pragma unroll
-                for (int k = 0; k < N; k++) A[k] = X[k];
+                for (int k = 0; k < N; k++) {
+                    if (k < kFsmKeep) keep[k * N2 + t] = X[k];  // this thread's slots only
+                    else A[k] = X[k];
+                }
             } else {
                 float* __restrict__ o = part + (size_t)item * N3 + ta + N2 * tb;
 #pragma unroll
-                for (int k = 0; k < N; k++) o[N * k] = A[k].x * X[k].x - A[k].y * X[k].y;
+                for (int k = 0; k < N; k++) {
+                    const float2 a = k < kFsmKeep ? keep[k * N2 + t] : A[k];
+                    o[N * k] = a.x * X[k].x - a.y * X[k].y;
+                }
             }
         }
     }
