@@ -890,7 +890,7 @@ size_t kbg_solve_bytes(const kbg_desc* d) {
     const size_t nv = d->nv;
     // f, face fluxes, w / prim / residual rows / dx / dy, geometry + walls + fw, velocity grid
     size_t b = C * 2 * nv * esz + F * 2 * nv * esz + C * (2 * 4 * 8 + 16 * 8 + 2 * 8) + F * (3 + 4 + 4) * 8 + 3 * nv * esz;
-    if (d->flux != KBG_KFVS) b += F * kbg::kFaceCoef * 8;
+    if (d->flux != KBG_KFVS) b += F * kbg::kFaceCoef * 8 + (esz == 4 ? F * kbg::kFaceC32 * 4 : 0);
     if (kbg::step2d_threads(d->precision == KBG_FP64, d->nv) == 0) b += C * 8 * 8 + C * 4 * nv * esz;  // slopes
     else if (esz == 4) b += 2 * nv * 4 + F * 8;  // departure form: reference Maxwellian, per-face coefficients
     return b;
@@ -1099,6 +1099,8 @@ int kbg_solve_create(kbg_ctx* ctx, const kbg_desc* desc, kbg_solve** out) {
             {&S.yff, FY * 2 * d.nv * s->esz},
             {(void**)&S.xcoef, coef ? FX * kbg::kFaceCoef * 8 : 0},
             {(void**)&S.ycoef, coef ? FY * kbg::kFaceCoef * 8 : 0},
+            {(void**)&S.xc32, coef && !s->fp64 ? FX * kbg::kFaceC32 * 4 : 0},
+            {(void**)&S.yc32, coef && !s->fp64 ? FY * kbg::kFaceC32 * 4 : 0},
             {(void**)&s->rcell, C * 16 * 8},
         };
         for (auto& b : bufs) {

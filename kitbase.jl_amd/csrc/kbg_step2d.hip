@@ -114,13 +114,14 @@ __global__ __launch_bounds__(64) void k_face2d(const Solve2D* __restrict__ solve
     const Solve2D& S = solves[si];
     const int nx = S.nx, ny = S.ny;
     const int nxf = (nx - 1) * ny;
-    int dir, fi, cL, cR;
+    int dir, fi, cL, cR, pa;
     if (b < nxf) {
         dir = 0;
         const int i = b % (nx - 1) + 2, j = b / (nx - 1) + 1;
         fi = (i - 1) + (nx + 1) * (j - 1);
         cL = (i - 2) + nx * (j - 1);
         cR = cL + 1;
+        pa = i - 1;
     } else {
         b -= nxf;
         dir = 1;
@@ -128,6 +129,7 @@ __global__ __launch_bounds__(64) void k_face2d(const Solve2D* __restrict__ solve
         fi = (i - 1) + nx * (j - 1);
         cL = (i - 1) + nx * (j - 2);
         cR = cL + nx;
+        pa = j - 1;
     }
     const double* geo = (dir == 0 ? S.xgeo : S.ygeo) + (size_t)fi * 3;
     const double n1 = geo[0], n2 = geo[1];
@@ -152,14 +154,59 @@ __global__ __launch_bounds__(64) void k_face2d(const Solve2D* __restrict__ solve
     local_frame4(wl, n1, n2, WL);
     local_frame4(wr, n1, n2, WR);
     double* rec = (dir == 0 ? S.xcoef : S.ycoef) + (size_t)fi * kFaceCoef;
+    float* c32 = S.xc32 ? (dir == 0 ? S.xc32 : S.yc32) + (size_t)fi * kFaceC32 : nullptr;
+    const double len = geo[2];
     if constexpr (FLUX == KCU) {
         KcuFace<4> F;
         kcu_face<4>(WL, WR, S.K, S.gam, S.mu, S.om, S.dt, F);
         *(KcuFace<4>*)rec = F;
+        if (c32) {
+            // the values k_flux2d would form from the record, with the same expressions
+            for (int m = 0; m < kFaceC32; m++) c32[m] = 0.f;
+            c32[0] = float(F.prim[0] * (F.prim[3] / kPi));
+            c32[1] = float(F.prim[3]);
+            c32[2] = float(F.prim[1]);
+            c32[3] = float(F.prim[2]);
+            c32[4] = float(S.K / (2.0 * F.prim[3]));
+            c32[6] = float(F.Mt1 * len);
+            c32[7] = float(F.Mt2 * len);
+        }
     } else {
         UgksFace2 F;
         ugks_face_2d(WL, WR, S.K, S.gam, S.mu, S.om, S.dt, dxL, dxR, F);
         *(UgksFace2*)rec = F;
+        if (c32) {
+            const double kbd = S.K / (2.0 * F.prim[3]);
+            c32[0] = float(F.prim[0] * (F.prim[3] / kPi));
+            c32[1] = float(F.prim[3]);
+            c32[2] = float(F.prim[1]);
+            c32[3] = float(F.prim[2]);
+            c32[4] = float(kbd);
+            for (int m = 0; m < 5; m++) c32[5 + m] = float(F.Mt[m] * len);
+            for (int m = 0; m < 4; m++) {
+                c32[10 + m] = float(F.aL[m]);
+                c32[14 + m] = float(F.aR[m]);
+                c32[18 + m] = float(F.aT[m]);
+            }
+            c32[22] = float(0.5 * F.aL[3] * kbd);
+            c32[23] = float(0.5 * F.aL[3] * F.Mxi2);
+            c32[24] = float(0.5 * F.aR[3] * kbd);
+            c32[25] = float(0.5 * F.aR[3] * F.Mxi2);
+            c32[26] = float(0.5 * F.aT[3] * kbd);
+            c32[27] = float(0.5 * F.aT[3] * F.Mxi2);
+        }
+    }
+    if (c32) {
+        // the two cells' slope coefficients and half widths, as k_flux2d's slope_c forms them
+        const int n = dir == 0 ? nx : ny, step = dir == 0 ? 1 : nx;
+        const SlopeC<float> sa = slope_c<float>(S.interp2, d, cL, pa, n, step);
+        const SlopeC<float> sb = slope_c<float>(S.interp2, d, cR, pa + 1, n, step);
+        c32[28] = sa.rL;
+        c32[29] = sa.rR;
+        c32[30] = sb.rL;
+        c32[31] = sb.rR;
+        c32[32] = float(0.5 * d[cL]);
+        c32[33] = float(0.5 * d[cR]);
     }
 }
 
@@ -270,9 +317,25 @@ __global__ __launch_bounds__(flux2d_max_threads<T>()) void k_flux2d(const Solve2
         // ---- interior face between cells a (low side) and a + step
         const double* d = dir == 0 ? S.dx : S.dy;
         const int bc_ = a + step;
-        const T dxa = T(0.5 * d[a]), dxb = T(0.5 * d[bc_]);
-        const SlopeC<T> sa = slope_c<T>(S.interp2, d, a, pa, n, step);
-        const SlopeC<T> sb = slope_c<T>(S.interp2, d, bc_, pa + 1, n, step);
+        // FP32 KCU / UGKS solves: the face's coefficients as k_face2d wrote them once per face
+        const float* c32 = nullptr;
+        if constexpr (sizeof(T) == 4 && FLUX != KFVS)
+            if (S.xc32) c32 = (dir == 0 ? S.xc32 : S.yc32) + (size_t)fi * kFaceC32;
+        T dxa, dxb;
+        SlopeC<T> sa, sb;
+        if (c32) {
+            const int ka = !S.interp2 ? SK_NONE : (pa == 1 ? SK_RIGHT : (pa == n ? SK_LEFT : SK_LIM));
+            const int kb2 = !S.interp2 ? SK_NONE : (pa + 1 == 1 ? SK_RIGHT : (pa + 1 == n ? SK_LEFT : SK_LIM));
+            sa = SlopeC<T>{ka, T(c32[28]), T(c32[29])};
+            sb = SlopeC<T>{kb2, T(c32[30]), T(c32[31])};
+            dxa = T(c32[32]);
+            dxb = T(c32[33]);
+        } else {
+            dxa = T(0.5 * d[a]);
+            dxb = T(0.5 * d[bc_]);
+            sa = slope_c<T>(S.interp2, d, a, pa, n, step);
+            sb = slope_c<T>(S.interp2, d, bc_, pa + 1, n, step);
+        }
         const int am = pa > 1 ? a - step : a, bp = pa + 1 < n ? bc_ + step : bc_;
         const int oam = am * RS * E, oa = a * RS * E, ob = bc_ * RS * E, obp = bp * RS * E;
         const int lim = S.limiter;
@@ -284,6 +347,29 @@ __global__ __launch_bounds__(flux2d_max_threads<T>()) void k_flux2d(const Solve2
         if constexpr (FLUX == KFVS) {
             M0 = T(dt * len);
             M1 = T(0.5 * dt * dt * len);
+        } else if (c32) {
+            A = T(c32[0]);
+            lam = T(c32[1]);
+            Ui = T(c32[2]);
+            Vi = T(c32[3]);
+            kb = T(c32[4]);
+            M0 = T(c32[5]);
+            M1 = T(c32[6]);
+            M2 = T(c32[7]);
+            M3 = T(c32[8]);
+            M4 = T(c32[9]);
+#pragma unroll
+            for (int m = 0; m < 4; m++) {
+                aL[m] = T(c32[10 + m]);
+                aR[m] = T(c32[14 + m]);
+                aT[m] = T(c32[18 + m]);
+            }
+            c1L = T(c32[22]);
+            c2L = T(c32[23]);
+            c1R = T(c32[24]);
+            c2R = T(c32[25]);
+            c1T = T(c32[26]);
+            c2T = T(c32[27]);
         } else {
             if constexpr (FLUX == KCU) {
                 const KcuFace<4>& F = *(const KcuFace<4>*)rec;

@@ -77,6 +77,8 @@ struct Solve2D {
     void* yff;
     double* xcoef;                                 // [(nx+1)*ny][kFaceCoef] KCU / UGKS face chain records
     double* ycoef;                                 // [nx*(ny+1)][kFaceCoef]
+    float* xc32;                                   // FP32 KCU / UGKS solves: [(nx+1)*ny][kFaceC32] the face
+    float* yc32;                                   //   flux's FP32 coefficients (k_face2d writes, k_flux2d reads)
     double* rcell;                                 // [nx*ny][16] residual rows
     int* health;                                   // failure flags (kbg_device.h prim_health)
     // FP32 solves of the fused step store f - Mr, Mr a per-solve reference Maxwellian (h and b), and
@@ -122,6 +124,10 @@ struct SolveUS {
 
 // doubles per face-coefficient record (a UgksFace2 or a KcuFace<4>, padded)
 constexpr int kFaceCoef = 32;
+// FP32 coefficient record of an interior face: A, lambda, U, V, K / (2 lambda), Mt[0..4] len,
+// aL[4], aR[4], aT[4], the six aX[3] products of the UGKS flux, the reciprocal centre spacings of
+// the two cells (rL, rR each) and their half widths
+constexpr int kFaceC32 = 40;
 // Threads per workgroup of the fused 2D step (one face or cell per workgroup): 64 ceil(nv / 256), 4
 // velocities per thread, at most 1024 in FP32 and 512 in FP64 (the register budget of an FP64
 // thread); 0 when the velocity grid is larger (the phase kernels then step the solve)
