@@ -1,4 +1,4 @@
-"""The workloads of this build as KitBase config dicts (the keys of KitBase's config files,
+"""The benchmark and test workloads as KitBase config dicts (the keys of KitBase's config files,
 read_dict io_read.jl:95-128).
 
 sod_1d1f1v      Sod shock tube, 1D1F1V BGK + KFVS, 100 cells x 64 Newton-Cotes velocities on [-5, 5],

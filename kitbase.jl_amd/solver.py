@@ -232,7 +232,7 @@ class SolverSet:
                          boundary=bnd if isinstance(bnd, list) else bnd, cfl=cfg["cfl"], maxTime=cfg["maxTime"],
                          hasForce=cfg.get("hasForce", False))
         if self.set.nSpecies != 1:
-            raise NotImplementedError("multi-species solves are outside this build")
+            raise NotImplementedError("multi-species solves are not supported (the single-species time step only)")
         sp = self.set.space
         Dx, Dv = int(sp[0]), int(sp[4])
         if Dx == 1:
@@ -522,7 +522,7 @@ def make_desc(ks: SolverSet, precision: str = "fp64") -> tuple[_lib.KbgDesc, lis
     bcs = ks.boundaries()
     for k, b in enumerate(bcs):
         if b not in _lib.BC:
-            raise NotImplementedError(f"boundary {b!r} is outside this build (fix and maxwell are supported)")
+            raise NotImplementedError(f"boundary {b!r} is not supported (fix and maxwell are)")
         d.bc[k] = _lib.BC[b]
     g = ks.gas
     d.K, d.gamma, d.mu_ref, d.omega, d.Pr = g.K, g.gamma, g.mu_ref, g.omega, g.Pr

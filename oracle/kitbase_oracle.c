@@ -4,7 +4,7 @@
  * Shakhov step!).  It also restates the setup helpers the step depends on (velocity quadratures,
  * the structured 2D mesh geometry).
  *
- * TEST INFRASTRUCTURE ONLY: this is the checker for the HIP path.  Only tests/,
+ * TEST INFRASTRUCTURE ONLY: the HIP path is tested against it.  Only tests/,
  * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it; the product library never
  * does.  OpenMP threads the cell and face loops the way KitBase threads them with @threads, so the
  * same code doubles as the CPU baseline.
@@ -1285,7 +1285,7 @@ void orc_step_2f2v(double *w, double *prim, double *h, double *b, const double *
 }
 
 /* ------------------------------------------------------------------------------------------ */
-/* 1D drivers                                                                                   */
+/* 1D whole solves                                                                             */
 /* ------------------------------------------------------------------------------------------ */
 
 static int nw_1d(const orc_solver1d *S) { return S->vdim == 3 ? 5 : 3; }
@@ -1446,7 +1446,7 @@ int orc_run_1d(orc_solver1d *S, double dt, int nsteps, double *res) {
 }
 
 /* ------------------------------------------------------------------------------------------ */
-/* 2D drivers (2F2V)                                                                            */
+/* 2D whole solves (2F2V)                                                                      */
 /* ------------------------------------------------------------------------------------------ */
 
 #define C2(i, j) ((size_t)((i) - 1) + (size_t)S->nx * ((j) - 1))

@@ -1,7 +1,7 @@
 /*
  * kitbase_oracle.h -- FP64 CPU restatement of KitBase.jl's discrete-velocity time step.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path: only tests/,
+ * TEST INFRASTRUCTURE ONLY.  The HIP path is tested against this library: only tests/,
  * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The product library
  * (libkbgpu.so) never links, loads or calls it.
  *
@@ -137,7 +137,7 @@ void orc_step_2f2v(double *w, double *prim, double *h, double *b, const double *
                    const double *v, const double *weights, int n, double K, double gam, double mu,
                    double om, double Pr, double ds, double dt, double *RES, double *AVG, int coll);
 
-/* ---- whole-solve drivers: 1D (1F1V, 2F1V, 1F3V) ----
+/* ---- whole solves: 1D (1F1V, 2F1V, 1F3V) ----
  * Cell storage index k = i - 1 + ng for the physical 1-based index i in [1-ng, nx+ng].
  * Face storage index i - 1 for the physical face i in [1, nx+1] (between cells i-1 and i).
  * f is [cell][nf][nv]; w and prim are [cell][nw]; sw [cell][nw]; sf [cell][nf][nv];

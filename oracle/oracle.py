@@ -1,6 +1,6 @@
 """ctypes binding of the FP64 CPU oracle (oracle/liboracle.so, built from kitbase_oracle.c).
 
-TEST INFRASTRUCTURE ONLY: the checker for the HIP path.  Only tests/, __graft_entry__.smoke() and
+TEST INFRASTRUCTURE ONLY: the HIP path is tested against it.  Only tests/, __graft_entry__.smoke() and
 bench.py's cpu_baseline leg may import this module.  The oracle restates KitBase.jl v0.9.31 (Julia,
 which is not installed here and cannot be built or run), function by function; the citations are
 in kitbase_oracle.h / kitbase_oracle.c.

@@ -1049,7 +1049,7 @@ def test_step_1f2v_bgk_fixed_point_relaxation_and_shakhov_refused():
 
 
 def test_cavity_1f2v_symmetry_and_mass():
-    """The 2D1F2V driver (one distribution): walls at rest and a bump symmetric under x <-> y keep the
+    """The 2D1F2V whole solve (one distribution): walls at rest and a bump symmetric under x <-> y keep the
     transpose symmetry and the total mass over 10 steps, KFVS and KCU."""
     n = 10
     for flux in ("kfvs", "kcu"):
